@@ -1,0 +1,158 @@
+/*
+ * lumen_ex.h -- liblumen extension ABI: the splat rasterization path of the
+ * reference's Mojo core (src/raster/pipeline.mojo:48-231) on MI355X.
+ *
+ * Plain C, caller-owned device pointers, LumenStatus returns, the caller's
+ * HIP stream (hipStream_t passed as void*, NULL = default stream).  No torch
+ * types.  A context owns the per-frame workspace (grown on demand, never
+ * shrunk), so steady-state calls allocate nothing.  One context serves one
+ * stream at a time; distinct contexts are independent.
+ *
+ * Callers: the Python SDK re-pointed through ctypes (lumen_amd/_lib.py), and
+ * the Mojo core through external_call (see INTEGRATION.md for both bindings).
+ */
+#ifndef LUMEN_EX_H
+#define LUMEN_EX_H
+
+#include "lumen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUMEN_EX_ABI_VERSION 1
+
+/* Status values beyond LumenStatus (c_api/include/lumen.h:29-36). */
+#define LUMEN_EX_ERROR_DEVICE ((LumenStatus)-100)   /* a HIP call failed */
+#define LUMEN_EX_ERROR_CAPACITY ((LumenStatus)-101) /* a reserved capacity was exceeded */
+
+typedef struct LumenExContext LumenExContext;
+
+/* Gaussian scene as device planes; replaces GaussianSoA
+ * (src/scene/gaussian.mojo:47-107): 11 planes of `count` floats and
+ * sh = count*48 floats indexed g*48 + k*3 + c.  opacity is activated (not a
+ * logit), rotation is (w, x, y, z) and need not be normalized. */
+typedef struct {
+    uint64_t count;
+    const float* pos_x;
+    const float* pos_y;
+    const float* pos_z;
+    const float* scale_x;
+    const float* scale_y;
+    const float* scale_z;
+    const float* rot_w;
+    const float* rot_x;
+    const float* rot_y;
+    const float* rot_z;
+    const float* opacity;
+    const float* sh;
+} LumenExScene;
+
+/* CameraConfig (src/core/config.mojo:17-42) + PinholeCamera pose
+ * (src/scene/camera.mojo:37-40).  rotation is camera-to-world (w, x, y, z);
+ * the camera looks along +Z with +Y down in view space
+ * (src/scene/camera.mojo:69-79). */
+typedef struct {
+    uint32_t width;
+    uint32_t height;
+    float focal_x;
+    float focal_y;
+    float principal_x;
+    float principal_y;
+    float near_plane;
+    float far_plane;
+    float position[3];
+    float rotation[4];
+} LumenExCamera;
+
+/* RenderConfig fields the reference reads (src/core/config.mojo:65-102). */
+typedef struct {
+    uint32_t tile_width;     /* 16 */
+    uint32_t tile_height;    /* 16; tile_width*tile_height must be 64..1024, a multiple of 64 */
+    float lowpass_filter;    /* 0.3 */
+    int32_t max_sh_degree;   /* 0..3 */
+    int32_t white_background;
+    int32_t compute_depth;
+    uint32_t flags;          /* LUMEN_EX_* below */
+    uint32_t batch_depth;    /* compositor LDS batch: 0 (= 256), 64, 128 or 256 */
+} LumenExRenderConfig;
+
+#define LUMEN_EX_KEEP_STAGES 1u   /* keep the unsorted keys/values for lumen_ex_copy_stage */
+#define LUMEN_EX_SAVE_PIXELS 2u   /* keep final transmittance and evaluation counts per pixel */
+
+typedef struct {
+    uint64_t visible_splats;        /* RenderOutput.visible_splats (pipeline.mojo:159) */
+    uint64_t total_splat_instances; /* RenderOutput.total_splat_instances (pipeline.mojo:160) */
+    uint64_t tiles_x;
+    uint64_t tiles_y;
+    uint32_t sort_passes;           /* 8-bit digit passes actually run */
+    uint32_t reserved;
+} LumenExRenderStats;
+
+LumenExContext* lumen_ex_context_create(int device);
+void lumen_ex_context_destroy(LumenExContext* ctx);
+int lumen_ex_abi_version(void);
+
+/* rasterize_scene (src/raster/pipeline.mojo:48-231).  out_rgb: H*W*3 floats,
+ * out_depth: H*W floats (device).  Every pixel is written: empty tiles get the
+ * background (0, or 1 with white_background) and depth 0.  Blocks the host
+ * once per call to learn the instance count. */
+LumenStatus lumen_ex_render(LumenExContext* ctx, const LumenExScene* scene,
+                            const LumenExCamera* camera, const LumenExRenderConfig* config,
+                            float* out_rgb, float* out_depth, LumenExRenderStats* out_stats,
+                            void* stream);
+
+/* radix_sort_64bit (src/raster/radix_sort.mojo:39-87): stable ascending sort
+ * of n 64-bit keys with 32-bit values, in place on device memory.  Digits
+ * that are equal across all keys are skipped (a stable pass over them is the
+ * identity). */
+LumenStatus lumen_ex_radix_sort(LumenExContext* ctx, uint64_t* keys, uint32_t* values,
+                                uint64_t n, void* stream);
+
+/* TileRangeIndex.build_from_sorted_keys (src/raster/binning.mojo:306-328):
+ * ranges[2*t], ranges[2*t+1] = [start, end) of tile t in the sorted keys, or
+ * (-1, -1) when the tile has no instance. */
+LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint32_t tiles,
+                                  int32_t* ranges, void* stream);
+
+/* Stage buffers of the last lumen_ex_render on this context, copied to host
+ * memory (synchronous).  bytes must equal the stage's size. */
+typedef enum {
+    LUMEN_EX_STAGE_VIS_TO_ORIG = 0,   /* int32  [V]   visible -> original index */
+    LUMEN_EX_STAGE_RECORDS = 1,       /* float  [V][12] cx cy i00 2*i01 | i11 op r g | b depth 0 0 */
+    LUMEN_EX_STAGE_RADIUS = 2,        /* int32  [V]   3-sigma radius */
+    LUMEN_EX_STAGE_KEYS_UNSORTED = 3, /* uint64 [M]   (needs LUMEN_EX_KEEP_STAGES) */
+    LUMEN_EX_STAGE_VALS_UNSORTED = 4, /* uint32 [M]   (needs LUMEN_EX_KEEP_STAGES) */
+    LUMEN_EX_STAGE_KEYS = 5,          /* uint64 [M]   sorted */
+    LUMEN_EX_STAGE_VALS = 6,          /* uint32 [M]   sorted */
+    LUMEN_EX_STAGE_RANGES = 7,        /* int32  [T][2] */
+    LUMEN_EX_STAGE_T_FINAL = 8,       /* float  [H*W] (needs LUMEN_EX_SAVE_PIXELS) */
+    LUMEN_EX_STAGE_N_EVAL = 9         /* int32  [H*W] (needs LUMEN_EX_SAVE_PIXELS) */
+} LumenExStage;
+
+LumenStatus lumen_ex_copy_stage(LumenExContext* ctx, int stage, void* dst_host, size_t bytes);
+
+/* Per-stage GPU timing with HIP events recorded on the render's stream around
+ * each stage of lumen_ex_render (off by default).  lumen_ex_stage_times waits
+ * for the last render's events and writes up to n durations in ms, in the
+ * order of LumenExTimedStage. */
+typedef enum {
+    LUMEN_EX_TIME_COUNT = 0,     /* k_count + compaction scan */
+    LUMEN_EX_TIME_EMIT = 1,      /* records + keys */
+    LUMEN_EX_TIME_SORT = 2,      /* all radix passes */
+    LUMEN_EX_TIME_RANGES = 3,    /* tile ranges */
+    LUMEN_EX_TIME_COMPOSITE = 4, /* compositor */
+    LUMEN_EX_TIME_STAGES = 5
+} LumenExTimedStage;
+
+LumenStatus lumen_ex_set_profiling(LumenExContext* ctx, int enable);
+LumenStatus lumen_ex_stage_times(LumenExContext* ctx, float* ms_out, int n);
+
+/* Message of the last failing lumen_ex_* call on this thread ("" if none). */
+const char* lumen_ex_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUMEN_EX_H */

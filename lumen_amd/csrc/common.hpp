@@ -1,0 +1,142 @@
+// common.hpp -- error reporting, device buffers and wave64 primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "lumen_ex.h"
+
+namespace lumen {
+
+// Last error of the calling thread (lumen_ex_last_error).
+void set_error(const char* fmt, ...);
+const char* last_error();
+void clear_error();
+
+inline LumenStatus fail(LumenStatus s, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  set_error("%s", buf);
+  return s;
+}
+
+#define LUMEN_HIP(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::lumen::fail(LUMEN_EX_ERROR_DEVICE, "%s failed: %s (%s:%d)", #expr,          \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                     \
+  } while (0)
+
+#define LUMEN_TRY(expr)                      \
+  do {                                       \
+    LumenStatus s_ = (expr);                 \
+    if (s_ != LUMEN_SUCCESS) return s_;      \
+  } while (0)
+
+#define LUMEN_LAUNCHED() LUMEN_HIP(hipGetLastError())
+
+// A device allocation that only grows.  hipFree on growth synchronizes the
+// device, which is acceptable because growth stops after the first frames.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+inline LumenStatus ensure(DevBuf& b, size_t bytes) {
+  if (bytes <= b.cap && b.ptr) return LUMEN_SUCCESS;
+  if (b.ptr) {
+    LUMEN_HIP(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.cap = 0;
+  }
+  size_t want = bytes + bytes / 4 + 256;
+  hipError_t e = hipMalloc(&b.ptr, want);
+  if (e != hipSuccess) {
+    b.ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(LUMEN_ERROR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+  }
+  b.cap = want;
+  return LUMEN_SUCCESS;
+}
+
+inline void release(DevBuf& b) {
+  if (b.ptr) (void)hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.cap = 0;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ---- wave64 / block primitives ------------------------------------------
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = threadIdx.x & 63u;
+  return (1ull << lane) - 1ull;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Exclusive scan across the block; every thread must call it.  s_wave must
+// hold THREADS/64 elements.  Ends with a barrier so s_wave may be reused.
+template <typename T, int THREADS>
+__device__ __forceinline__ T block_exclusive_scan(T x, T* s_wave, T& total) {
+  constexpr int NW = THREADS / 64;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const T inc = wave_inclusive_scan(x);
+  if (lane == 63) s_wave[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    T s = lane < NW ? s_wave[lane] : T(0);
+    s = wave_inclusive_scan(s);
+    if (lane < NW) s_wave[lane] = s;
+  }
+  __syncthreads();
+  const T prefix = w > 0 ? s_wave[w - 1] : T(0);
+  total = s_wave[NW - 1];
+  __syncthreads();
+  return prefix + inc - x;
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, T v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <typename T>
+inline LumenStatus fill(T* p, T v, uint64_t n, hipStream_t st) {
+  if (n == 0) return LUMEN_SUCCESS;
+  hipLaunchKernelGGL(k_fill<T>, dim3(div_up(n, 256)), dim3(256), 0, st, p, v, n);
+  LUMEN_LAUNCHED();
+  return LUMEN_SUCCESS;
+}
+
+}  // namespace lumen

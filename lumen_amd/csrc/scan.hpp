@@ -1,0 +1,111 @@
+// scan.hpp -- device-wide exclusive prefix sum (reduce / spine / downsweep).
+//
+// Used for the visible-splat compaction (packed visible flag + tile count per
+// Gaussian, 64-bit) and for the radix sort's digit-major block histograms
+// (32-bit).  Loads and stores are coalesced; the per-thread serial part runs
+// out of LDS.
+#pragma once
+
+#include "common.hpp"
+
+namespace lumen {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restrict__ in, uint64_t n,
+                                                              T* __restrict__ partials) {
+  __shared__ T s_w[kScanThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const uint64_t idx = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) t += s_w[w];
+    partials[blockIdx.x] = t;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void k_scan_spine(T* __restrict__ partials, uint32_t nb,
+                                                     T* __restrict__ total_out) {
+  __shared__ T s_w[1024 / 64];
+  T carry = 0;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const T x = i < nb ? partials[i] : T(0);
+    T tot;
+    const T ex = block_exclusive_scan<T, 1024>(x, s_w, tot);
+    if (i < nb) partials[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict__ in, uint64_t n,
+                                                            const T* __restrict__ partials,
+                                                            T* __restrict__ out) {
+  __shared__ T s[kScanChunk + kScanChunk / 32];
+  __shared__ T s_w[kScanThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int li = j * kScanThreads + t;
+    const uint64_t idx = base + li;
+    s[scan_pad(li)] = idx < n ? in[idx] : T(0);
+  }
+  __syncthreads();
+  T v[kScanItems];
+  T sum = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = s[scan_pad(t * kScanItems + j)];
+    sum += v[j];
+  }
+  T tot;
+  T ex = block_exclusive_scan<T, kScanThreads>(sum, s_w, tot) + partials[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    s[scan_pad(t * kScanItems + j)] = ex;
+    ex += v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int li = j * kScanThreads + t;
+    const uint64_t idx = base + li;
+    if (idx < n) out[idx] = s[scan_pad(li)];
+  }
+}
+
+// Exclusive scan of n elements (in may equal out).  partials must hold
+// div_up(n, kScanChunk) elements; total_out (device, may be null) receives
+// the sum of all elements.
+template <typename T>
+inline LumenStatus exclusive_scan(const T* in, T* out, uint64_t n, T* partials, T* total_out,
+                                  hipStream_t st) {
+  const uint32_t nb = div_up(n ? n : 1, kScanChunk);
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials);
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_spine<T>, dim3(1), dim3(1024), 0, st, partials, nb, total_out);
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials, out);
+  LUMEN_LAUNCHED();
+  return LUMEN_SUCCESS;
+}
+
+}  // namespace lumen

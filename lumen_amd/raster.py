@@ -1,0 +1,198 @@
+"""rasterize_scene on MI355X: the Python face of lumen_ex_render.
+
+Mirrors the reference's interface (src/raster/pipeline.mojo:21-231,
+src/core/config.mojo:65-102): RenderConfig with the same field names and
+defaults, RenderOutput with rgb / depth buffers and the visible_splats /
+total_splat_instances counters, rasterize_scene(cloud, camera, config).
+Buffers are torch tensors on the GPU; all computation happens in liblumen.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .camera import PinholeCamera
+from .scene import DeviceScene, GaussianScene
+
+
+@dataclass
+class RenderConfig:
+    tile_width: int = 16
+    tile_height: int = 16
+    lowpass_filter: float = 0.3
+    max_sh_degree: int = 3
+    white_background: bool = False
+    compute_depth: bool = True
+    batch_depth: int = 256  # compositor LDS batch (64, 128, 256)
+
+    def ex_struct(self, flags: int = 0):
+        c = _lib.LumenExRenderConfig()
+        c.tile_width = self.tile_width
+        c.tile_height = self.tile_height
+        c.lowpass_filter = self.lowpass_filter
+        c.max_sh_degree = self.max_sh_degree
+        c.white_background = int(bool(self.white_background))
+        c.compute_depth = int(bool(self.compute_depth))
+        c.flags = flags
+        c.batch_depth = self.batch_depth
+        return c
+
+    def tiles(self, width: int, height: int):
+        return ((width + self.tile_width - 1) // self.tile_width,
+                (height + self.tile_height - 1) // self.tile_height)
+
+
+def camera_struct(camera: PinholeCamera):
+    c = _lib.LumenExCamera()
+    cfg = camera.config
+    c.width, c.height = cfg.width, cfg.height
+    c.focal_x, c.focal_y = cfg.focal_x, cfg.focal_y
+    c.principal_x, c.principal_y = cfg.principal_x, cfg.principal_y
+    c.near_plane, c.far_plane = cfg.near_plane, cfg.far_plane
+    for k in range(3):
+        c.position[k] = float(camera.position[k])
+    for k in range(4):
+        c.rotation[k] = float(camera.rotation[k])
+    return c
+
+
+@dataclass
+class RenderOutput:
+    width: int
+    height: int
+    rgb: "object"    # torch.Tensor (H, W, 3) float32
+    depth: "object"  # torch.Tensor (H, W) float32
+    visible_splats: int
+    total_splat_instances: int
+    sort_passes: int = 0
+
+    def get_pixel_rgb(self, x: int, y: int):
+        if 0 <= x < self.width and 0 <= y < self.height:
+            return tuple(float(v) for v in self.rgb[y, x].tolist())
+        return (0.0, 0.0, 0.0)
+
+
+def _stream_handle(device) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class Rasterizer:
+    """Owns one lumen_ex context (device workspace that grows, never shrinks)."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = _lib.lib().lumen_ex_context_create(device)
+        if not self._ctx:
+            raise RuntimeError("lumen_ex_context_create failed: " +
+                               _lib.lib().lumen_ex_last_error().decode())
+        self.device = device
+        self._last = None
+
+    def close(self):
+        if self._ctx:
+            _lib.lib().lumen_ex_context_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def render(self, scene, camera: PinholeCamera, config: Optional[RenderConfig] = None,
+               keep_stages: bool = False, save_pixels: bool = False,
+               out_rgb=None, out_depth=None) -> RenderOutput:
+        import torch
+        config = config or RenderConfig()
+        if isinstance(scene, GaussianScene):
+            scene = scene.to_device(f"cuda:{self.device}")
+        dev = scene.device
+        W, H = camera.config.width, camera.config.height
+        if out_rgb is None:
+            out_rgb = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+        if out_depth is None:
+            out_depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+        assert out_rgb.is_contiguous() and out_rgb.shape == (H, W, 3)
+        assert out_depth.is_contiguous() and out_depth.shape == (H, W)
+        flags = (_lib.LUMEN_EX_KEEP_STAGES if keep_stages else 0) | \
+                (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0)
+        sc = scene.ex_struct()
+        cam = camera_struct(camera)
+        cfg = config.ex_struct(flags)
+        stats = _lib.LumenExRenderStats()
+        st = _lib.lib().lumen_ex_render(self._ctx, ctypes.byref(sc), ctypes.byref(cam),
+                                        ctypes.byref(cfg), out_rgb.data_ptr(),
+                                        out_depth.data_ptr(), ctypes.byref(stats),
+                                        _stream_handle(dev))
+        _lib.check("lumen_ex_render", st)
+        self._last = (int(stats.visible_splats), int(stats.total_splat_instances),
+                      int(stats.tiles_x * stats.tiles_y), W * H)
+        return RenderOutput(W, H, out_rgb, out_depth, int(stats.visible_splats),
+                            int(stats.total_splat_instances), int(stats.sort_passes))
+
+    def stage(self, name: str) -> np.ndarray:
+        """Host copy of a stage buffer of the last render (see lumen_ex.h)."""
+        V, M, T, P = self._last
+        shapes = {"vis_to_orig": (np.int32, (V,)), "records": (np.float32, (V, 12)),
+                  "radius": (np.int32, (V,)), "keys_unsorted": (np.uint64, (M,)),
+                  "vals_unsorted": (np.uint32, (M,)), "keys": (np.uint64, (M,)),
+                  "vals": (np.uint32, (M,)), "ranges": (np.int32, (T, 2)),
+                  "t_final": (np.float32, (P,)), "n_eval": (np.int32, (P,))}
+        dt, shape = shapes[name]
+        out = np.empty(shape, dt)
+        st = _lib.lib().lumen_ex_copy_stage(self._ctx, _lib.STAGES[name],
+                                            out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
+        _lib.check("lumen_ex_copy_stage", st)
+        return out
+
+    def set_profiling(self, enable: bool) -> None:
+        _lib.check("lumen_ex_set_profiling",
+                   _lib.lib().lumen_ex_set_profiling(self._ctx, int(bool(enable))))
+
+    def stage_times(self) -> dict:
+        """GPU ms per stage of the last render (HIP events on its stream)."""
+        n = len(_lib.TIMED_STAGES)
+        buf = (ctypes.c_float * n)()
+        _lib.check("lumen_ex_stage_times", _lib.lib().lumen_ex_stage_times(self._ctx, buf, n))
+        return {k: float(buf[i]) for i, k in enumerate(_lib.TIMED_STAGES)}
+
+    def radix_sort(self, keys, values) -> None:
+        """In-place stable sort of int64-stored uint64 keys with int32 values."""
+        import torch
+        assert keys.dtype == torch.int64 and values.dtype == torch.int32
+        assert keys.is_contiguous() and values.is_contiguous() and keys.numel() == values.numel()
+        st = _lib.lib().lumen_ex_radix_sort(self._ctx, keys.data_ptr(), values.data_ptr(),
+                                            keys.numel(), _stream_handle(keys.device))
+        _lib.check("lumen_ex_radix_sort", st)
+
+
+def build_ranges(sorted_keys, tiles: int):
+    import torch
+    ranges = torch.empty((tiles, 2), dtype=torch.int32, device=sorted_keys.device)
+    st = _lib.lib().lumen_ex_build_ranges(sorted_keys.data_ptr(), sorted_keys.numel(), tiles,
+                                          ranges.data_ptr(), _stream_handle(sorted_keys.device))
+    _lib.check("lumen_ex_build_ranges", st)
+    return ranges
+
+
+_default: Optional[Rasterizer] = None
+
+
+def default_rasterizer() -> Rasterizer:
+    global _default
+    if _default is None:
+        _default = Rasterizer(0)
+    return _default
+
+
+def rasterize_scene(cloud, camera: PinholeCamera, config: Optional[RenderConfig] = None) -> RenderOutput:
+    """rasterize_scene(cloud, camera, config) (src/raster/pipeline.mojo:48)."""
+    return default_rasterizer().render(cloud, camera, config)

@@ -1,0 +1,127 @@
+/*
+ * lumen_oracle.h -- CPU oracle for the splat rasterization path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a scalar, single-threaded C restatement
+ * of the reference's Mojo rasterizer (src/raster/pipeline.mojo:48-231 and the
+ * math it calls).  It is the checker for the HIP implementation in lumen_amd/
+ * and the CPU baseline that bench.py times.  Only tests/, the smoke check in
+ * __graft_entry__.py and bench.py's cpu_baseline leg may load it; the product
+ * path (liblumen.so and the lumen_amd Python modules) never links, loads or calls it.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fno-fast-math).  No FMA
+ * contraction, correctly rounded '/' and sqrt, IEEE binary32 everywhere the
+ * reference uses Float32 and int64 where it uses Int.
+ */
+#ifndef LUMEN_ORACLE_H
+#define LUMEN_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CameraConfig (src/core/config.mojo:17-42) + PinholeCamera pose
+ * (src/scene/camera.mojo:37-40).  rot = (w, x, y, z) camera-to-world. */
+typedef struct {
+    int64_t width, height;
+    float focal_x, focal_y, principal_x, principal_y;
+    float near_plane, far_plane;
+    float pos[3];
+    float rot[4];
+} orc_camera;
+
+/* RenderConfig fields the reference reads (src/core/config.mojo:65-102,
+ * src/raster/pipeline.mojo:60,104,138,184,221,228). */
+typedef struct {
+    int64_t tile_width, tile_height;
+    float lowpass_filter;
+    int64_t max_sh_degree;
+    int32_t white_background;
+    int32_t compute_depth;
+    /* 0: exp from orc_expf (the Cody-Waite + Taylor-7 exp the HIP kernel
+     *    also implements); 1: libm expf.  Mojo's std.math.exp is neither;
+     *    both agree with it to ~1 ulp. */
+    int32_t exp_mode;
+} orc_config;
+
+/* Scene in the GaussianSoA layout (src/scene/gaussian.mojo:47-107):
+ * 11 planes of n floats, sh = n*48 floats with index g*48 + k*3 + c. */
+typedef struct {
+    int64_t n;
+    const float *pos_x, *pos_y, *pos_z;
+    const float *scale_x, *scale_y, *scale_z;
+    const float *rot_w, *rot_x, *rot_y, *rot_z;
+    const float *opacity;
+    const float *sh;
+} orc_scene;
+
+/* Everything rasterize_scene computes, stage by stage. */
+typedef struct {
+    int64_t n, visible, instances, tiles, tiles_x, tiles_y, evaluations;
+    int64_t width, height;
+    /* per visible splat (visible index order) */
+    int64_t *vis_to_orig;  /* visible -> original Gaussian index */
+    float *proj;           /* V x 10: cx, cy, depth, inv00, inv01, inv11, r, g, b, opacity */
+    int64_t *radius;       /* V */
+    int64_t *rect;         /* V x 4: min_tx, min_ty, max_tx, max_ty */
+    /* per instance */
+    uint64_t *keys_unsorted;
+    int64_t *vals_unsorted;
+    uint64_t *keys;        /* sorted */
+    int64_t *vals;         /* sorted */
+    /* per tile: start, end ([-1,-1] when empty) */
+    int64_t *ranges;
+    /* per pixel */
+    float *rgb;            /* H x W x 3 */
+    float *depth;          /* H x W */
+    float *t_final;        /* H x W transmittance after the last blend */
+    int32_t *n_eval;       /* H x W alpha evaluations incl. the terminating one */
+} orc_result;
+
+/* ---- whole path ------------------------------------------------------- */
+orc_result *orc_rasterize(const orc_scene *scene, const orc_camera *cam,
+                          const orc_config *cfg);
+void orc_result_free(orc_result *r);
+
+/* ---- single stages (known-answer tests and sort/range parity) ---------- */
+float orc_expf(float x);
+void orc_expf_array(const float *x, float *y, int64_t n);
+void orc_quat_normalized(const float q[4], float out[4]);
+void orc_quat_to_matrix(const float q[4], float out[9]);
+void orc_world_to_cam(const orc_camera *cam, const float p[3], float out[3]);
+void orc_cam_to_screen(const orc_camera *cam, const float pc[3], float out[2]);
+void orc_cov3d(const float scale[3], const float q[4], float out[6]);
+void orc_jacobian(const float t[3], float fx, float fy, float out[6]);
+void orc_ewa(const float cov3d[6], const float pc[3], const float view_rot[9],
+             float fx, float fy, float lowpass, float out[3]);
+int64_t orc_eigen(const float cov2d[3], float out_lambda[2]);
+void orc_inverse2d(const float cov2d[3], float out[3]);
+void orc_sh_basis(const float dir[3], float out[16]);
+void orc_sh_color(const float *sh48, const float dir[3], int64_t degree, float out[3]);
+int orc_depth_visible(float near_plane, float far_plane, float z);
+int orc_opacity_visible(float op);
+int orc_screen_box_visible(int64_t w, int64_t h, float cx, float cy, int64_t r);
+int orc_tile_overlap(int64_t w, int64_t h, int64_t tw, int64_t th, float cx,
+                     float cy, int64_t r, int64_t out[4]);
+uint64_t orc_encode_key(uint32_t tile, float depth);
+void orc_radix_sort(uint64_t *keys, int64_t *vals, int64_t n);
+void orc_build_ranges(const uint64_t *keys, int64_t n, int64_t tiles, int64_t *ranges);
+float orc_pixel_alpha(float px, float py, float cx, float cy, const float inv[3],
+                      float opacity, int32_t exp_mode);
+/* PixelAccumulator.blend over a list; state = {r, g, b, T, depth}. Returns
+ * the index of the terminating splat or -1. */
+int64_t orc_blend_sequence(const float *colors, const float *alphas,
+                           const float *depths, int64_t n, float state[5]);
+
+/* ---- losses (src/optim/loss.mojo) -------------------------------------- */
+float orc_l1_loss(const float *pred, const float *target, int64_t n, float *grad);
+float orc_ssim(const float *pred, const float *target, int64_t w, int64_t h, int64_t win);
+float orc_psnr(const float *pred, const float *target, int64_t n);
+float orc_combined_loss(float l1, float ssim, float lambda_dssim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

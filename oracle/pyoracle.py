@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU oracle (oracle/liblumen_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker / CPU baseline.  The product path
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_PATH = os.path.join(_HERE, "liblumen_oracle.so")
+REF_PATH = os.path.join(_HERE, "_ref", "liblumen_ref.so")
+
+f32p = ctypes.POINTER(ctypes.c_float)
+i64p = ctypes.POINTER(ctypes.c_int64)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class OrcCamera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int64), ("height", ctypes.c_int64),
+                ("focal_x", ctypes.c_float), ("focal_y", ctypes.c_float),
+                ("principal_x", ctypes.c_float), ("principal_y", ctypes.c_float),
+                ("near_plane", ctypes.c_float), ("far_plane", ctypes.c_float),
+                ("pos", ctypes.c_float * 3), ("rot", ctypes.c_float * 4)]
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [("tile_width", ctypes.c_int64), ("tile_height", ctypes.c_int64),
+                ("lowpass_filter", ctypes.c_float), ("max_sh_degree", ctypes.c_int64),
+                ("white_background", ctypes.c_int32), ("compute_depth", ctypes.c_int32),
+                ("exp_mode", ctypes.c_int32)]
+
+
+class OrcScene(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64)] + [(k, f32p) for k in (
+        "pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x", "rot_y",
+        "rot_z", "opacity", "sh")]
+
+
+class OrcResult(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in (
+        "n", "visible", "instances", "tiles", "tiles_x", "tiles_y", "evaluations", "width",
+        "height")] + [
+        ("vis_to_orig", i64p), ("proj", f32p), ("radius", i64p), ("rect", i64p),
+        ("keys_unsorted", u64p), ("vals_unsorted", i64p), ("keys", u64p), ("vals", i64p),
+        ("ranges", i64p), ("rgb", f32p), ("depth", f32p), ("t_final", f32p), ("n_eval", i32p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"{ORACLE_PATH} missing: run `make -C oracle`")
+        L = ctypes.CDLL(ORACLE_PATH)
+        L.orc_rasterize.restype = ctypes.POINTER(OrcResult)
+        L.orc_rasterize.argtypes = [ctypes.POINTER(OrcScene), ctypes.POINTER(OrcCamera),
+                                    ctypes.POINTER(OrcConfig)]
+        L.orc_result_free.argtypes = [ctypes.POINTER(OrcResult)]
+        L.orc_expf.restype = ctypes.c_float
+        L.orc_expf.argtypes = [ctypes.c_float]
+        L.orc_expf_array.restype = None
+        L.orc_expf_array.argtypes = [f32p, f32p, ctypes.c_int64]
+        for name, args in {
+            "orc_quat_normalized": [f32p, f32p], "orc_quat_to_matrix": [f32p, f32p],
+            "orc_world_to_cam": [ctypes.POINTER(OrcCamera), f32p, f32p],
+            "orc_cam_to_screen": [ctypes.POINTER(OrcCamera), f32p, f32p],
+            "orc_cov3d": [f32p, f32p, f32p],
+            "orc_jacobian": [f32p, ctypes.c_float, ctypes.c_float, f32p],
+            "orc_ewa": [f32p, f32p, f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float, f32p],
+            "orc_inverse2d": [f32p, f32p], "orc_sh_basis": [f32p, f32p],
+            "orc_sh_color": [f32p, f32p, ctypes.c_int64, f32p],
+            "orc_radix_sort": [u64p, i64p, ctypes.c_int64],
+            "orc_build_ranges": [u64p, ctypes.c_int64, ctypes.c_int64, i64p],
+        }.items():
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = args
+        L.orc_eigen.restype = ctypes.c_int64
+        L.orc_eigen.argtypes = [f32p, f32p]
+        L.orc_depth_visible.restype = ctypes.c_int
+        L.orc_depth_visible.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.orc_opacity_visible.restype = ctypes.c_int
+        L.orc_opacity_visible.argtypes = [ctypes.c_float]
+        L.orc_screen_box_visible.restype = ctypes.c_int
+        L.orc_screen_box_visible.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_int64]
+        L.orc_tile_overlap.restype = ctypes.c_int
+        L.orc_tile_overlap.argtypes = [ctypes.c_int64] * 4 + [ctypes.c_float, ctypes.c_float,
+                                                              ctypes.c_int64, i64p]
+        L.orc_encode_key.restype = ctypes.c_uint64
+        L.orc_encode_key.argtypes = [ctypes.c_uint32, ctypes.c_float]
+        L.orc_pixel_alpha.restype = ctypes.c_float
+        L.orc_pixel_alpha.argtypes = [ctypes.c_float] * 4 + [f32p, ctypes.c_float, ctypes.c_int32]
+        L.orc_blend_sequence.restype = ctypes.c_int64
+        L.orc_blend_sequence.argtypes = [f32p, f32p, f32p, ctypes.c_int64, f32p]
+        L.orc_l1_loss.restype = ctypes.c_float
+        L.orc_l1_loss.argtypes = [f32p, f32p, ctypes.c_int64, f32p]
+        L.orc_ssim.restype = ctypes.c_float
+        L.orc_ssim.argtypes = [f32p, f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
+        L.orc_psnr.restype = ctypes.c_float
+        L.orc_psnr.argtypes = [f32p, f32p, ctypes.c_int64]
+        L.orc_combined_loss.restype = ctypes.c_float
+        L.orc_combined_loss.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        _lib = L
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(f32p)
+
+
+def f32(*vals) -> np.ndarray:
+    return np.ascontiguousarray(np.array(vals, np.float32).ravel())
+
+
+def camera_struct(camera) -> OrcCamera:
+    c = OrcCamera()
+    cfg = camera.config
+    c.width, c.height = cfg.width, cfg.height
+    c.focal_x, c.focal_y = cfg.focal_x, cfg.focal_y
+    c.principal_x, c.principal_y = cfg.principal_x, cfg.principal_y
+    c.near_plane, c.far_plane = cfg.near_plane, cfg.far_plane
+    for k in range(3):
+        c.pos[k] = float(camera.position[k])
+    for k in range(4):
+        c.rot[k] = float(camera.rotation[k])
+    return c
+
+
+def config_struct(config, exp_mode: int = 0) -> OrcConfig:
+    c = OrcConfig()
+    c.tile_width, c.tile_height = config.tile_width, config.tile_height
+    c.lowpass_filter = config.lowpass_filter
+    c.max_sh_degree = config.max_sh_degree
+    c.white_background = int(bool(config.white_background))
+    c.compute_depth = int(bool(config.compute_depth))
+    c.exp_mode = exp_mode
+    return c
+
+
+def rasterize(scene, camera, config, exp_mode: int = 0) -> dict:
+    """Run the oracle's rasterize_scene; returns numpy copies of every stage."""
+    L = lib()
+    planes = np.ascontiguousarray(scene.planes, np.float32)
+    sh = np.ascontiguousarray(scene.sh, np.float32)
+    n = planes.shape[1]
+    sc = OrcScene()
+    sc.n = n
+    names = ("pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x",
+             "rot_y", "rot_z", "opacity")
+    rows = [np.ascontiguousarray(planes[k]) for k in range(11)]
+    for k, name in enumerate(names):
+        setattr(sc, name, fptr(rows[k]))
+    sc.sh = fptr(sh)
+    cam = camera_struct(camera)
+    cfg = config_struct(config, exp_mode)
+    rp = L.orc_rasterize(ctypes.byref(sc), ctypes.byref(cam), ctypes.byref(cfg))
+    r = rp.contents
+    V, M, T = r.visible, r.instances, r.tiles
+    W, H = r.width, r.height
+
+    def arr(ptr, count, dtype):
+        if count == 0:
+            return np.zeros((0,), dtype)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dtype, copy=True)
+
+    out = {
+        "visible": V, "instances": M, "tiles": T, "tiles_x": r.tiles_x, "tiles_y": r.tiles_y,
+        "evaluations": r.evaluations,
+        "vis_to_orig": arr(r.vis_to_orig, V, np.int64),
+        "proj": arr(r.proj, V * 10, np.float32).reshape(V, 10),
+        "radius": arr(r.radius, V, np.int64),
+        "rect": arr(r.rect, V * 4, np.int64).reshape(V, 4),
+        "keys_unsorted": arr(r.keys_unsorted, M, np.uint64),
+        "vals_unsorted": arr(r.vals_unsorted, M, np.int64),
+        "keys": arr(r.keys, M, np.uint64),
+        "vals": arr(r.vals, M, np.int64),
+        "ranges": arr(r.ranges, 2 * T, np.int64).reshape(T, 2),
+        "rgb": arr(r.rgb, W * H * 3, np.float32).reshape(H, W, 3),
+        "depth": arr(r.depth, W * H, np.float32).reshape(H, W),
+        "t_final": arr(r.t_final, W * H, np.float32).reshape(H, W),
+        "n_eval": arr(r.n_eval, W * H, np.int32).reshape(H, W),
+    }
+    L.orc_result_free(rp)
+    return out
+
+
+def radix_sort(keys: np.ndarray, vals: np.ndarray):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.int64).copy()
+    lib().orc_radix_sort(k.ctypes.data_as(u64p), v.ctypes.data_as(i64p), k.size)
+    return k, v
+
+
+def build_ranges(keys: np.ndarray, tiles: int) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint64)
+    r = np.empty((tiles, 2), np.int64)
+    lib().orc_build_ranges(k.ctypes.data_as(u64p), k.size, tiles, r.ctypes.data_as(i64p))
+    return r
+
+
+def expf(x: float) -> float:
+    return lib().orc_expf(x)
+
+
+def expf_array(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_expf_array(fptr(x), fptr(y), x.size)
+    return y
