@@ -83,6 +83,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="random1m_1920x1080")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-backward", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -135,8 +136,46 @@ def main() -> int:
     stage_ms = {k: v / args.steps for k, v in acc.items()}
 
     work = stage_work(N, V, M, T, 0, passes, E)
-    work["sort"] = ("hbm", 8.0 * M + 24.0 * passes * M)
-    stages = {k: roofline_entry(b, w, stage_ms[k]) for k, (b, w) in work.items() if stage_ms[k] > 0}
+    stages = {k: roofline_entry(b, w, stage_ms[k]) for k, (b, w) in work.items()
+              if stage_ms.get(k, 0) > 0}
+
+    # forward + backward: L1 + D-SSIM against a render of a copy with means
+    # perturbed by N(0, 0.01), gradients for every parameter plane
+    fb = None
+    if not args.no_backward:
+        from lumen_amd.scene import perturbed
+        from lumen_amd.training import combined_loss, rasterize
+        with torch.no_grad():
+            target = R.render(perturbed(scene, 0.01, 7).to_device("cuda:0"), cam, cfg).rgb.clone()
+        planes = dev.planes.clone().requires_grad_(True)
+        shp = dev.sh.clone().requires_grad_(True)
+
+        def train_step():
+            planes.grad = None
+            shp.grad = None
+            img, _ = rasterize(planes, shp, cam, cfg, rasterizer=R)
+            loss = combined_loss(img, target)
+            loss.backward()
+            return loss
+
+        for _ in range(args.warmup):
+            train_step()
+        torch.cuda.synchronize()
+        R.set_profiling(True)
+        acc_b = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            train_step()
+            for k, v in R.stage_times().items():
+                acc_b[k] = acc_b.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        R.set_profiling(False)
+        ms_fb = (t1 - t0) * 1e3 / args.steps
+        fb = {"value": round(1000.0 / ms_fb, 2), "unit": "fwd+bwd steps/s",
+              "ms_per_step": round(ms_fb, 4),
+              "loss": "0.8 L1 + 0.2 (1 - SSIM 8x8), target = render of means + N(0, 0.01)",
+              "stage_ms": {k: round(v / args.steps, 5) for k, v in acc_b.items()}}
     dominant = max(stages, key=lambda k: stages[k]["avg_ms"])
     roofline = dict(stages[dominant])
     roofline["kernel"] = {"composite": "k_composite", "sort": "k_sort_hist+k_sort_scatter",
@@ -161,6 +200,7 @@ def main() -> int:
                    "sort_passes": passes},
         "roofline": roofline,
         "stages": stages,
+        "fwd_bwd": fb,
     }
 
     if not args.no_cpu_baseline:

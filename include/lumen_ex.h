@@ -127,10 +127,48 @@ typedef enum {
     LUMEN_EX_STAGE_VALS = 6,          /* uint32 [M]   sorted */
     LUMEN_EX_STAGE_RANGES = 7,        /* int32  [T][2] */
     LUMEN_EX_STAGE_T_FINAL = 8,       /* float  [H*W] (needs LUMEN_EX_SAVE_PIXELS) */
-    LUMEN_EX_STAGE_N_EVAL = 9         /* int32  [H*W] (needs LUMEN_EX_SAVE_PIXELS) */
+    LUMEN_EX_STAGE_N_EVAL = 9,        /* int32  [H*W] (needs LUMEN_EX_SAVE_PIXELS) */
+    LUMEN_EX_STAGE_GRAD_VIS = 10      /* float  [V][10] of the last backward: dL/d mean x, mean y,
+                                         inv00, inv01, inv11, opacity, r, g, b, depth */
 } LumenExStage;
 
 LumenStatus lumen_ex_copy_stage(LumenExContext* ctx, int stage, void* dst_host, size_t bytes);
+
+/* Gradient planes in the layout of GaussianGradients
+ * (src/optim/gradients.mojo:10-106), device pointers, ACCUMULATED into (the
+ * caller zeroes them, as GaussianGradients.zero does).  viewspace_grad_2d and
+ * observation_count follow accumulate_viewspace_grad (gradients.mojo:96-99):
+ * every Gaussian visible in the view gets |dL/d mean2d| (pixels) added and
+ * its count incremented.  Either of the two may be NULL. */
+typedef struct {
+    float* pos_x;
+    float* pos_y;
+    float* pos_z;
+    float* scale_x;
+    float* scale_y;
+    float* scale_z;
+    float* rot_w;
+    float* rot_x;
+    float* rot_y;
+    float* rot_z;
+    float* opacity;
+    float* sh;                  /* count*48 */
+    float* viewspace_grad_2d;   /* count */
+    int32_t* observation_count; /* count */
+} LumenExGradients;
+
+/* Backward of the last lumen_ex_render on this context, which must have
+ * been called with LUMEN_EX_SAVE_PIXELS and the same scene, camera and config.
+ * grad_rgb: H*W*3 floats, grad_depth: H*W floats or NULL (device).  The
+ * compositor's reverse pass recomputes every alpha exactly as the forward
+ * did, recovers transmittance by division from the saved final value, and
+ * accumulates per-splat gradients; a second kernel applies the EWA, SH and
+ * covariance adjoints per visible Gaussian. */
+LumenStatus lumen_ex_render_backward(LumenExContext* ctx, const LumenExScene* scene,
+                                     const LumenExCamera* camera,
+                                     const LumenExRenderConfig* config, const float* grad_rgb,
+                                     const float* grad_depth, LumenExGradients* grads,
+                                     void* stream);
 
 /* Per-stage GPU timing with HIP events recorded on the render's stream around
  * each stage of lumen_ex_render (off by default).  lumen_ex_stage_times waits
@@ -142,7 +180,9 @@ typedef enum {
     LUMEN_EX_TIME_SORT = 2,      /* all radix passes */
     LUMEN_EX_TIME_RANGES = 3,    /* tile ranges */
     LUMEN_EX_TIME_COMPOSITE = 4, /* compositor */
-    LUMEN_EX_TIME_STAGES = 5
+    LUMEN_EX_TIME_BWD_COMPOSITE = 5,  /* compositor reverse pass (lumen_ex_render_backward) */
+    LUMEN_EX_TIME_BWD_PREPROCESS = 6, /* EWA / SH / covariance adjoints */
+    LUMEN_EX_TIME_STAGES = 7
 } LumenExTimedStage;
 
 LumenStatus lumen_ex_set_profiling(LumenExContext* ctx, int enable);

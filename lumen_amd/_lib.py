@@ -85,11 +85,17 @@ class LumenExRenderStats(ctypes.Structure):
                 ("sort_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class LumenExGradients(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x", "rot_y",
+        "rot_z", "opacity", "sh", "viewspace_grad_2d", "observation_count")]
+
+
 LUMEN_EX_KEEP_STAGES = 1
 LUMEN_EX_SAVE_PIXELS = 2
 
 STAGES = {"vis_to_orig": 0, "records": 1, "radius": 2, "keys_unsorted": 3, "vals_unsorted": 4,
-          "keys": 5, "vals": 6, "ranges": 7, "t_final": 8, "n_eval": 9}
+          "keys": 5, "vals": 6, "ranges": 7, "t_final": 8, "n_eval": 9, "grad_vis": 10}
 
 LEGACY_SYMBOLS = [
     "lumen_get_version", "lumen_get_version_major", "lumen_get_version_minor",
@@ -142,11 +148,17 @@ _SIGS = {
                                              ctypes.c_void_p, ctypes.c_void_p]),
     "lumen_ex_copy_stage": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.c_size_t]),
+    "lumen_ex_render_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenExScene),
+                                                ctypes.POINTER(LumenExCamera),
+                                                ctypes.POINTER(LumenExRenderConfig),
+                                                ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.POINTER(LumenExGradients),
+                                                ctypes.c_void_p]),
     "lumen_ex_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lumen_ex_stage_times": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
 }
 
-TIMED_STAGES = ("count", "emit", "sort", "ranges", "composite")
+TIMED_STAGES = ("count", "emit", "sort", "ranges", "composite", "bwd_composite", "bwd_preprocess")
 
 _lib = None
 _lock = threading.Lock()

@@ -33,6 +33,7 @@ HIPCC_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
     "-Wno-unused-command-line-argument",
     "-I", INCLUDE,
     "-I", CSRC,

@@ -1,0 +1,445 @@
+// backward.hip -- gradient of the splat rasterization path (lumen_ex_render_backward).
+//
+// The reference has no backward (src/optim/trainer.mojo:59-65 injects dummy
+// gradients); the output layout is its GaussianGradients
+// (src/optim/gradients.mojo:10-106).  Two kernels:
+//
+//   k_composite_bwd   one workgroup per tile, one lane per pixel, walking the
+//                     tile's sorted list BACK to front from each pixel's last
+//                     evaluated splat.  Alphas are recomputed exactly as the
+//                     forward computed them (same expressions, same exp), so
+//                     the skip / clamp / termination decisions are the
+//                     forward's; transmittance before each blend is recovered
+//                     from the saved final value by division.  Per-splat
+//                     gradients (mean2d, inverse covariance, opacity, rgb,
+//                     depth) are summed over the wave with shuffles, over the
+//                     block in LDS, and flushed with one atomic per splat,
+//                     component and tile.
+//   k_preprocess_bwd  one thread per visible Gaussian: adjoints of the SH
+//                     colour (and view direction), the projection to screen,
+//                     the 2D inverse, the EWA projection, and the world
+//                     covariance R S S^T R^T with its quaternion
+//                     normalization; writes the Gaussian's gradient planes and
+//                     the view-space positional gradient norm for densification.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "common.hpp"
+#include "context.hpp"
+#include "lumen_ex.h"
+#include "splat_math.hpp"
+
+namespace lumen {
+
+constexpr int kGradVis = 10;  // mean x, mean y, inv00, inv01, inv11, opacity, r, g, b, depth
+
+template <int BATCH>
+__global__ __launch_bounds__(256) void k_composite_bwd(
+    const int2* __restrict__ ranges, const uint32_t* __restrict__ vals,
+    const float4* __restrict__ rec, ViewParams v, const float* __restrict__ t_final,
+    const int32_t* __restrict__ n_eval, const float* __restrict__ g_rgb,
+    const float* __restrict__ g_depth, float* __restrict__ grad_vis) {
+  __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+  __shared__ uint32_t s_vi[BATCH];
+  __shared__ float s_acc[BATCH * kGradVis];
+  __shared__ int s_last;
+  const int nthreads = v.tile_w * v.tile_h;
+  const int tile = blockIdx.x;
+  const int2 rg = ranges[tile];
+  if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
+  const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+  const int t = threadIdx.x;
+  const int px = tx * v.tile_w + t % v.tile_w;
+  const int py = ty * v.tile_h + t / v.tile_w;
+  const bool inside = px < v.width && py < v.height;
+  for (int e = t; e < BATCH * kGradVis; e += nthreads) s_acc[e] = 0.0f;
+  if (t == 0) s_last = rg.x - 1;
+  __syncthreads();
+  int last = rg.x - 1;
+  float T = 1.0f, gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gD = 0.0f;
+  if (inside) {
+    const size_t pix = (size_t)py * v.width + px;
+    last = rg.x + n_eval[pix] - 1;
+    T = t_final[pix];
+    gC0 = g_rgb[3 * pix + 0];
+    gC1 = g_rgb[3 * pix + 1];
+    gC2 = g_rgb[3 * pix + 2];
+    gD = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
+  }
+  atomicMax(&s_last, last);
+  __syncthreads();
+  const int block_last = s_last;
+  const float Tf = T;
+  const float gsum = gC0 + gC1 + gC2;
+  const float pxc = (float)px + 0.5f;
+  const float pyc = (float)py + 0.5f;
+  const int lane = t & 63;
+  float SC0 = 0.0f, SC1 = 0.0f, SC2 = 0.0f, SD = 0.0f;
+  for (int end = block_last + 1; end > rg.x; end -= BATCH) {
+    const int start = max(rg.x, end - BATCH);
+    const int n = end - start;
+    for (int j = t; j < n; j += nthreads) {
+      const uint32_t vi = vals[start + j];
+      s_vi[j] = vi;
+      s0[j] = rec[3u * vi + 0u];
+      s1[j] = rec[3u * vi + 1u];
+      s2[j] = rec[3u * vi + 2u];
+    }
+    __syncthreads();
+    for (int j = n - 1; j >= 0; --j) {
+      float gv[kGradVis];
+#pragma unroll
+      for (int k = 0; k < kGradVis; ++k) gv[k] = 0.0f;
+      bool contrib = false;
+      if (start + j <= last) {
+        const float4 a = s0[j];
+        const float4 b = s1[j];
+        const float dx = pxc - a.x;
+        const float dy = pyc - a.y;
+        float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
+        bool pclamp = false;
+        if (power < 0.0f) {
+          power = 0.0f;
+          pclamp = true;
+        }
+        if (!(power > 4.5f)) {
+          const float G = splat_exp(-power);
+          float alpha = b.y * G;
+          bool aclamp = false;
+          if (alpha > 0.99f) {
+            alpha = 0.99f;
+            aclamp = true;
+          }
+          if (!(alpha < kMinAlpha)) {
+            contrib = true;
+            const float4 c = s2[j];
+            const float om = 1.0f - alpha;
+            const float Tj = T / om;
+            const float w = alpha * Tj;
+            gv[6] = gC0 * w;
+            gv[7] = gC1 * w;
+            gv[8] = gC2 * w;
+            gv[9] = gD * w;
+            float dal = gC0 * (b.z * Tj - SC0 / om) + gC1 * (b.w * Tj - SC1 / om) +
+                        gC2 * (c.x * Tj - SC2 / om) + gD * (c.y * Tj - SD / om);
+            if (v.white_background) dal -= gsum * Tf / om;
+            SC0 += b.z * w;
+            SC1 += b.w * w;
+            SC2 += c.x * w;
+            SD += c.y * w;
+            T = Tj;
+            if (!aclamp) {
+              gv[5] = dal * G;
+              if (!pclamp) {
+                const float dp = -dal * b.y * G;
+                const float i01 = 0.5f * a.w;
+                gv[2] = dp * 0.5f * dx * dx;
+                gv[3] = dp * dx * dy;
+                gv[4] = dp * 0.5f * dy * dy;
+                gv[0] = -dp * (a.z * dx + i01 * dy);
+                gv[1] = -dp * (i01 * dx + b.x * dy);
+              }
+            }
+          }
+        }
+      }
+      if (__ballot(contrib)) {  // wave-uniform
+#pragma unroll
+        for (int k = 0; k < kGradVis; ++k) gv[k] = wave_sum(gv[k]);
+        if (lane == 0) {
+#pragma unroll
+          for (int k = 0; k < kGradVis; ++k) atomicAdd(&s_acc[j * kGradVis + k], gv[k]);
+        }
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < n * kGradVis; e += nthreads) {
+      const float x = s_acc[e];
+      if (x != 0.0f) atomicAdd(&grad_vis[(size_t)s_vi[e / kGradVis] * kGradVis + e % kGradVis], x);
+      s_acc[e] = 0.0f;
+    }
+    __syncthreads();
+  }
+}
+
+// dY_k/d(x, y, z) of the real SH polynomials of sh_basis, contracted with
+// dL/dY_k: returns dL/d(x, y, z) for a unit direction.
+__device__ __forceinline__ void sh_dir_grad(float x, float y, float z, const float dY[16], int ncoef,
+                                            float g[3]) {
+  const float xx = x * x, yy = y * y, zz = z * z;
+  float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  if (ncoef > 1) {
+    gy += -kShC1 * dY[1];
+    gz += kShC1 * dY[2];
+    gx += -kShC1 * dY[3];
+  }
+  if (ncoef > 4) {
+    gx += kShC2_0 * y * dY[4];
+    gy += kShC2_0 * x * dY[4];
+    gy += -kShC2_0 * z * dY[5];
+    gz += -kShC2_0 * y * dY[5];
+    gx += -2.0f * kShC2_1 * x * dY[6];
+    gy += -2.0f * kShC2_1 * y * dY[6];
+    gz += 4.0f * kShC2_1 * z * dY[6];
+    gx += -kShC2_0 * z * dY[7];
+    gz += -kShC2_0 * x * dY[7];
+    gx += 2.0f * kShC2_2 * x * dY[8];
+    gy += -2.0f * kShC2_2 * y * dY[8];
+  }
+  if (ncoef > 9) {
+    gx += -6.0f * kShC3_0 * x * y * dY[9];
+    gy += -kShC3_0 * (3.0f * xx - 3.0f * yy) * dY[9];
+    gx += kShC3_1 * y * z * dY[10];
+    gy += kShC3_1 * x * z * dY[10];
+    gz += kShC3_1 * x * y * dY[10];
+    gx += 2.0f * kShC3_2 * x * y * dY[11];
+    gy += -kShC3_2 * (4.0f * zz - xx - 3.0f * yy) * dY[11];
+    gz += -8.0f * kShC3_2 * y * z * dY[11];
+    gx += -6.0f * kShC3_3 * x * z * dY[12];
+    gy += -6.0f * kShC3_3 * y * z * dY[12];
+    gz += kShC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy) * dY[12];
+    gx += -kShC3_2 * (4.0f * zz - 3.0f * xx - yy) * dY[13];
+    gy += 2.0f * kShC3_2 * x * y * dY[13];
+    gz += -8.0f * kShC3_2 * x * z * dY[13];
+    gx += 2.0f * kShC3_4 * x * z * dY[14];
+    gy += -2.0f * kShC3_4 * y * z * dY[14];
+    gz += kShC3_4 * (xx - yy) * dY[14];
+    gx += -kShC3_0 * (3.0f * xx - 3.0f * yy) * dY[15];
+    gy += 6.0f * kShC3_0 * x * y * dY[15];
+  }
+  g[0] = gx;
+  g[1] = gy;
+  g[2] = gz;
+}
+
+struct GradPlanes {
+  float* p[11];
+  float* sh;
+  float* viewspace;
+  int32_t* count;
+};
+
+__global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v, uint32_t V,
+                                                        const int32_t* __restrict__ vis2orig,
+                                                        const float* __restrict__ grad_vis,
+                                                        GradPlanes out) {
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= V) return;
+  const uint32_t i = (uint32_t)vis2orig[k];
+  const float* g = grad_vis + (size_t)k * kGradVis;
+  const float gmx = g[0], gmy = g[1], ga = g[2], gb = g[3], gc = g[4];
+  const float* W = v.rot;
+  float gpos[3] = {0.0f, 0.0f, 0.0f};
+
+  // opacity
+  out.p[10][i] += g[5];
+
+  // colour = clamp(0.5 + sum_k sh_k Y_k(dir)), dir = normalize(pos - eye)
+  const float px = s.px[i], py = s.py[i], pz = s.pz[i];
+  float ux = px - v.eye[0], uy = py - v.eye[1], uz = pz - v.eye[2];
+  const float dl = sqrtf(ux * ux + uy * uy + uz * uz);
+  normalize3(ux, uy, uz);
+  float Y[16];
+  sh_basis(ux, uy, uz, Y);
+  const float* sh = s.sh + (size_t)i * 48u;
+  float raw[3] = {0.5f, 0.5f, 0.5f};
+  for (int kk = 0; kk < v.sh_coeffs; ++kk)
+    for (int ch = 0; ch < 3; ++ch) raw[ch] += sh[3 * kk + ch] * Y[kk];
+  float gcol[3];
+  for (int ch = 0; ch < 3; ++ch) gcol[ch] = (raw[ch] < 0.0f || raw[ch] > 1.0f) ? 0.0f : g[6 + ch];
+  float dY[16];
+  float* gsh = out.sh + (size_t)i * 48u;
+  for (int kk = 0; kk < 16; ++kk) {
+    dY[kk] = 0.0f;
+    if (kk >= v.sh_coeffs) continue;
+    for (int ch = 0; ch < 3; ++ch) {
+      gsh[3 * kk + ch] += gcol[ch] * Y[kk];
+      dY[kk] += gcol[ch] * sh[3 * kk + ch];
+    }
+  }
+  if (dl > 0.000001f) {
+    float gd[3];
+    sh_dir_grad(ux, uy, uz, dY, v.sh_coeffs, gd);
+    const float dot = gd[0] * ux + gd[1] * uy + gd[2] * uz;
+    gpos[0] += (gd[0] - ux * dot) / dl;
+    gpos[1] += (gd[1] - uy * dot) / dl;
+    gpos[2] += (gd[2] - uz * dot) / dl;
+  }
+
+  // camera-space position (same rotation as the forward)
+  const Quat pq{0.0f, px - v.eye[0], py - v.eye[1], pz - v.eye[2]};
+  const Quat r = quat_mul(quat_mul(v.q_view, pq), v.q_view_inv);
+  const float x = r.x, y = r.y;
+  const float z = r.z < 0.0001f ? 0.0001f : r.z;
+  const float fx = v.fx, fy = v.fy;
+  float gpc[3];
+  gpc[0] = gmx * fx / z;
+  gpc[1] = gmy * fy / z;
+  gpc[2] = g[9] - gmx * fx * x / (z * z) - gmy * fy * y / (z * z);
+  if (out.viewspace) out.viewspace[i] += sqrtf(gmx * gmx + gmy * gmy);
+  if (out.count) out.count[i] += 1;
+
+  // world covariance Sigma = M M^T, M = R(q) diag(s)
+  const Quat qraw{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
+  const float qn = sqrtf(quat_norm2(qraw));
+  const Quat qh = quat_normalized(qraw);
+  float R[9];
+  quat_matrix(qh, R);
+  float sc[3] = {s.sx[i], s.sy[i], s.sz[i]};
+  bool sclamp[3];
+  for (int a = 0; a < 3; ++a) {
+    sclamp[a] = sc[a] < 0.00001f;
+    if (sclamp[a]) sc[a] = 0.00001f;
+  }
+  float M[9];
+  for (int rr = 0; rr < 3; ++rr)
+    for (int cc = 0; cc < 3; ++cc) M[3 * rr + cc] = R[3 * rr + cc] * sc[cc];
+  float S[3][3];
+  for (int rr = 0; rr < 3; ++rr)
+    for (int cc = 0; cc < 3; ++cc)
+      S[rr][cc] = M[3 * rr] * M[3 * cc] + M[3 * rr + 1] * M[3 * cc + 1] + M[3 * rr + 2] * M[3 * cc + 2];
+
+  // T = J W and the screen covariance
+  const float itz = 1.0f / z, itz2 = itz * itz;
+  const float j00 = fx * itz, j02 = -fx * x * itz2, j11 = fy * itz, j12 = -fy * y * itz2;
+  float t0[3], t1[3], St0[3], St1[3];
+  for (int a = 0; a < 3; ++a) {
+    t0[a] = j00 * W[a] + j02 * W[6 + a];
+    t1[a] = j11 * W[3 + a] + j12 * W[6 + a];
+  }
+  for (int rr = 0; rr < 3; ++rr) {
+    St0[rr] = S[rr][0] * t0[0] + S[rr][1] * t0[1] + S[rr][2] * t0[2];
+    St1[rr] = S[rr][0] * t1[0] + S[rr][1] * t1[1] + S[rr][2] * t1[2];
+  }
+  const float c00 = t0[0] * St0[0] + t0[1] * St0[1] + t0[2] * St0[2] + v.lowpass;
+  const float c01 = t0[0] * St1[0] + t0[1] * St1[1] + t0[2] * St1[2];
+  const float c11 = t1[0] * St1[0] + t1[1] * St1[1] + t1[2] * St1[2] + v.lowpass;
+  const float det = c00 * c11 - c01 * c01;
+  float gc00 = 0.0f, gc01 = 0.0f, gc11 = 0.0f;
+  if (!(det > -0.0000001f && det < 0.0000001f)) {
+    const float id = 1.0f / det;
+    const float K = (-ga * c11 + gb * c01 - gc * c00) * id * id;
+    gc00 = gc * id + K * c11;
+    gc11 = ga * id + K * c00;
+    gc01 = -gb * id - 2.0f * K * c01;
+  }
+  float gt0[3], gt1[3], F[3][3];
+  for (int a = 0; a < 3; ++a) {
+    gt0[a] = 2.0f * gc00 * St0[a] + gc01 * St1[a];
+    gt1[a] = gc01 * St0[a] + 2.0f * gc11 * St1[a];
+  }
+  for (int rr = 0; rr < 3; ++rr)
+    for (int cc = 0; cc < 3; ++cc)
+      F[rr][cc] = gc00 * t0[rr] * t0[cc] + 0.5f * gc01 * (t0[rr] * t1[cc] + t1[rr] * t0[cc]) +
+                  gc11 * t1[rr] * t1[cc];
+  const float gj00 = gt0[0] * W[0] + gt0[1] * W[1] + gt0[2] * W[2];
+  const float gj02 = gt0[0] * W[6] + gt0[1] * W[7] + gt0[2] * W[8];
+  const float gj11 = gt1[0] * W[3] + gt1[1] * W[4] + gt1[2] * W[5];
+  const float gj12 = gt1[0] * W[6] + gt1[1] * W[7] + gt1[2] * W[8];
+  gpc[2] += gj00 * (-fx * itz2) + gj02 * (2.0f * fx * x * itz2 * itz) + gj11 * (-fy * itz2) +
+            gj12 * (2.0f * fy * y * itz2 * itz);
+  gpc[0] += gj02 * (-fx * itz2);
+  gpc[1] += gj12 * (-fy * itz2);
+  for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
+  out.p[0][i] += gpos[0];
+  out.p[1][i] += gpos[1];
+  out.p[2][i] += gpos[2];
+
+  // Sigma -> scale and rotation
+  float gM[9], gR[9];
+  for (int rr = 0; rr < 3; ++rr)
+    for (int cc = 0; cc < 3; ++cc)
+      gM[3 * rr + cc] = 2.0f * (F[rr][0] * M[cc] + F[rr][1] * M[3 + cc] + F[rr][2] * M[6 + cc]);
+  for (int cc = 0; cc < 3; ++cc) {
+    const float gs = gM[cc] * R[cc] + gM[3 + cc] * R[3 + cc] + gM[6 + cc] * R[6 + cc];
+    if (!sclamp[cc]) out.p[3 + cc][i] += gs;
+  }
+  for (int rr = 0; rr < 3; ++rr)
+    for (int cc = 0; cc < 3; ++cc) gR[3 * rr + cc] = gM[3 * rr + cc] * sc[cc];
+  const float w = qh.w, qx = qh.x, qy = qh.y, qz = qh.z;
+  float gq[4];
+  gq[0] = 2.0f * (-qz * gR[1] + qy * gR[2] + qz * gR[3] - qx * gR[5] - qy * gR[6] + qx * gR[7]);
+  gq[1] = 2.0f * (qy * gR[1] + qz * gR[2] + qy * gR[3] - 2.0f * qx * gR[4] - w * gR[5] +
+                  qz * gR[6] + w * gR[7] - 2.0f * qx * gR[8]);
+  gq[2] = 2.0f * (-2.0f * qy * gR[0] + qx * gR[1] + w * gR[2] + qx * gR[3] + qz * gR[5] -
+                  w * gR[6] + qz * gR[7] - 2.0f * qy * gR[8]);
+  gq[3] = 2.0f * (-2.0f * qz * gR[0] - w * gR[1] + qx * gR[2] + w * gR[3] - 2.0f * qz * gR[4] +
+                  qy * gR[5] + qx * gR[6] + qy * gR[7]);
+  if (qn > 0.000001f) {
+    const float dot = gq[0] * w + gq[1] * qx + gq[2] * qy + gq[3] * qz;
+    const float qq[4] = {w, qx, qy, qz};
+    for (int a = 0; a < 4; ++a) out.p[6 + a][i] += (gq[a] - qq[a] * dot) / qn;
+  }
+}
+
+}  // namespace lumen
+
+using namespace lumen;
+
+extern "C" LumenStatus lumen_ex_render_backward(LumenExContext* c, const LumenExScene* sc,
+                                                const LumenExCamera* cam,
+                                                const LumenExRenderConfig* cfg,
+                                                const float* grad_rgb, const float* grad_depth,
+                                                LumenExGradients* grads, void* stream) {
+  clear_error();
+  if (!c || !sc || !cam || !cfg || !grad_rgb || !grads)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_backward: null argument");
+  if (!c->pixels)
+    return fail(LUMEN_ERROR_UNINITIALIZED,
+                "lumen_ex_render_backward: the last render did not keep LUMEN_EX_SAVE_PIXELS");
+  if (sc->count != c->N || cam->width != c->width || cam->height != c->height ||
+      cfg->tile_width != c->tile_w || cfg->tile_height != c->tile_h)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT,
+                "lumen_ex_render_backward: scene/camera/config differ from the last render");
+  float* planes[11] = {grads->pos_x, grads->pos_y, grads->pos_z, grads->scale_x, grads->scale_y,
+                       grads->scale_z, grads->rot_w, grads->rot_x, grads->rot_y, grads->rot_z,
+                       grads->opacity};
+  for (float* p : planes)
+    if (!p) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_backward: null gradient plane");
+  if (!grads->sh) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_backward: null sh gradient");
+  hipStream_t st = (hipStream_t)stream;
+  LUMEN_HIP(hipSetDevice(c->device));
+  const uint64_t V = c->V;
+  c->has_grad = true;
+  if (V == 0 || c->M == 0) return LUMEN_SUCCESS;
+  ViewParams v;
+  fill_view(v, cam, cfg);
+  LUMEN_TRY(ensure(c->grad_vis, V * kGradVis * sizeof(float)));
+  LUMEN_TRY(fill<float>(c->grad_vis.as<float>(), 0.0f, V * kGradVis, st));
+  const uint32_t threads = cfg->tile_width * cfg->tile_height;
+  const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
+  const int2* ranges = c->ranges.as<int2>();
+  const uint32_t* vals = c->vals[c->final_buf].as<uint32_t>();
+  const float4* rec = c->rec.as<float4>();
+  const float* tf = c->t_final.as<float>();
+  const int32_t* ne = c->n_eval.as<int32_t>();
+  float* gv = c->grad_vis.as<float>();
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
+  switch (batch) {
+    case 64:
+      hipLaunchKernelGGL(k_composite_bwd<64>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
+                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_composite_bwd<128>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
+                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      break;
+    default:
+      hipLaunchKernelGGL(k_composite_bwd<256>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
+                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      break;
+  }
+  LUMEN_LAUNCHED();
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, true, st));
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, false, st));
+  GradPlanes gp;
+  for (int k = 0; k < 11; ++k) gp.p[k] = planes[k];
+  gp.sh = grads->sh;
+  gp.viewspace = grads->viewspace_grad_2d;
+  gp.count = grads->observation_count;
+  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(V, 256)), dim3(256), 0, st, scene_dev(sc), v,
+                     (uint32_t)V, c->vis2orig.as<int32_t>(), gv, gp);
+  LUMEN_LAUNCHED();
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, true, st));
+  return LUMEN_SUCCESS;
+}

@@ -25,6 +25,7 @@
 #include "scan.hpp"
 #include "sort.hpp"
 #include "splat_math.hpp"
+#include "context.hpp"
 
 namespace lumen {
 
@@ -40,17 +41,6 @@ void set_error(const char* fmt, ...) {
 }
 const char* last_error() { return t_last_error.c_str(); }
 void clear_error() { t_last_error.clear(); }
-
-struct SceneDev {
-  const float *px, *py, *pz, *sx, *sy, *sz, *rw, *rx, *ry, *rz, *op, *sh;
-  uint32_t n;
-};
-
-__device__ __forceinline__ bool load_geometry(const SceneDev& s, const ViewParams& v, uint32_t i,
-                                              SplatGeom& g) {
-  return splat_geometry(v, s.px[i], s.py[i], s.pz[i], s.sx[i], s.sy[i], s.sz[i],
-                        Quat{s.rw[i], s.rx[i], s.ry[i], s.rz[i]}, s.op[i], g);
-}
 
 __global__ void k_init_bounds(uint32_t* b) {
   b[0] = 0xFFFFFFFFu;
@@ -300,33 +290,6 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
 
 using namespace lumen;
 
-struct LumenExContext {
-  int device = 0;
-  DevBuf packed, part, totals, bounds;
-  DevBuf rec, vis2orig, radius;
-  DevBuf keys[2], vals[2], keys_keep, vals_keep;
-  DevBuf hist, hist_part;
-  DevBuf ranges, t_final, n_eval, minmax;
-  void* pinned = nullptr;
-  // metadata of the last render (for lumen_ex_copy_stage)
-  uint64_t V = 0, M = 0, T = 0, P = 0;
-  int final_buf = 0;
-  bool kept = false, pixels = false;
-  // optional per-stage timing (lumen_ex_set_profiling)
-  bool profile = false;
-  bool events_ready = false;
-  bool stage_ran[LUMEN_EX_TIME_STAGES] = {};
-  hipEvent_t ev[2 * LUMEN_EX_TIME_STAGES] = {};
-};
-
-// Records the start (end = false) or end of a timed stage on the stream.
-static LumenStatus mark(LumenExContext* c, int stage, bool end, hipStream_t st) {
-  if (!c->profile) return LUMEN_SUCCESS;
-  LUMEN_HIP(hipEventRecord(c->ev[2 * stage + (end ? 1 : 0)], st));
-  if (end) c->stage_ran[stage] = true;
-  return LUMEN_SUCCESS;
-}
-
 static bool plane_ok(const float* p) { return p != nullptr; }
 
 static LumenStatus composite_launch(uint32_t batch, uint32_t tiles, uint32_t threads,
@@ -385,25 +348,7 @@ static LumenStatus render_impl(LumenExContext* c, const LumenExScene* sc, const 
   LUMEN_HIP(hipSetDevice(c->device));
 
   ViewParams v;
-  memset(&v, 0, sizeof(v));
-  v.width = (int32_t)cam->width;
-  v.height = (int32_t)cam->height;
-  v.tile_w = (int32_t)tw;
-  v.tile_h = (int32_t)th;
-  v.tiles_x = (int32_t)((cam->width + tw - 1) / tw);
-  v.tiles_y = (int32_t)((cam->height + th - 1) / th);
-  v.fx = cam->focal_x;
-  v.fy = cam->focal_y;
-  v.cx = cam->principal_x;
-  v.cy = cam->principal_y;
-  v.near_plane = cam->near_plane;
-  v.far_plane = cam->far_plane;
-  v.lowpass = cfg->lowpass_filter;
-  static const int kCoeffs[4] = {1, 4, 9, 16};
-  v.sh_coeffs = kCoeffs[cfg->max_sh_degree];
-  v.white_background = cfg->white_background ? 1 : 0;
-  v.compute_depth = cfg->compute_depth ? 1 : 0;
-  make_view(v, cam->position, cam->rotation);
+  fill_view(v, cam, cfg);
 
   const uint32_t N = (uint32_t)sc->count;
   const uint32_t T = (uint32_t)v.tiles_x * (uint32_t)v.tiles_y;
@@ -497,6 +442,10 @@ static LumenStatus render_impl(LumenExContext* c, const LumenExScene* sc, const 
     ne = c->n_eval.as<int32_t>();
     c->pixels = true;
   }
+  // rasterize_scene returns its zero-filled RenderOutput before the
+  // background fill when there is no tile instance (pipeline.mojo:63-65,
+  // 162-163), so an empty frame is black even with white_background.
+  if (M == 0) v.white_background = 0;
   const uint32_t* vals_sorted = M > 0 ? c->vals[final_buf].as<uint32_t>() : nullptr;
   const float4* rec = M > 0 ? c->rec.as<float4>() : nullptr;
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, false, st));
@@ -508,7 +457,13 @@ static LumenStatus render_impl(LumenExContext* c, const LumenExScene* sc, const 
   c->M = M;
   c->T = T;
   c->P = P;
+  c->N = N;
+  c->width = cam->width;
+  c->height = cam->height;
+  c->tile_w = tw;
+  c->tile_h = th;
   c->final_buf = final_buf;
+  c->has_grad = false;
   if (stats) {
     stats->visible_splats = V;
     stats->total_splat_instances = M;
@@ -556,7 +511,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->bounds, &c->rec, &c->vis2orig,
                    &c->radius, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
                    &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part, &c->ranges,
-                   &c->t_final, &c->n_eval, &c->minmax};
+                   &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis};
   for (DevBuf* b : all) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   delete c;
@@ -649,6 +604,9 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t 
     case LUMEN_EX_STAGE_N_EVAL:
       if (!c->pixels) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_SAVE_PIXELS)");
       src = c->n_eval.ptr; want = c->P * 4; break;
+    case LUMEN_EX_STAGE_GRAD_VIS:
+      if (!c->has_grad) return fail(LUMEN_ERROR_UNINITIALIZED, "no backward since the last render");
+      src = c->grad_vis.ptr; want = c->V * 40; break;
     default: return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_copy_stage: unknown stage %d", stage);
   }
   if (bytes != want)

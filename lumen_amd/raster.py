@@ -91,6 +91,7 @@ class Rasterizer:
                                _lib.lib().lumen_ex_last_error().decode())
         self.device = device
         self._last = None
+        self.frame_id = 0  # incremented by every render on this context
 
     def close(self):
         if self._ctx:
@@ -133,6 +134,7 @@ class Rasterizer:
                                         out_depth.data_ptr(), ctypes.byref(stats),
                                         _stream_handle(dev))
         _lib.check("lumen_ex_render", st)
+        self.frame_id += 1
         self._last = (int(stats.visible_splats), int(stats.total_splat_instances),
                       int(stats.tiles_x * stats.tiles_y), W * H)
         return RenderOutput(W, H, out_rgb, out_depth, int(stats.visible_splats),
@@ -141,7 +143,8 @@ class Rasterizer:
     def stage(self, name: str) -> np.ndarray:
         """Host copy of a stage buffer of the last render (see lumen_ex.h)."""
         V, M, T, P = self._last
-        shapes = {"vis_to_orig": (np.int32, (V,)), "records": (np.float32, (V, 12)),
+        shapes = {"grad_vis": (np.float32, (V, 10)),
+                  "vis_to_orig": (np.int32, (V,)), "records": (np.float32, (V, 12)),
                   "radius": (np.int32, (V,)), "keys_unsorted": (np.uint64, (M,)),
                   "vals_unsorted": (np.uint32, (M,)), "keys": (np.uint64, (M,)),
                   "vals": (np.uint32, (M,)), "ranges": (np.int32, (T, 2)),
@@ -152,6 +155,21 @@ class Rasterizer:
                                             out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
         _lib.check("lumen_ex_copy_stage", st)
         return out
+
+    def render_backward(self, scene, camera: PinholeCamera, config: RenderConfig, grad_rgb,
+                        grad_depth, grads: "GaussianGradients") -> None:
+        """Accumulate the gradient of the last render (made with save_pixels=True
+        from the same scene/camera/config) into `grads`."""
+        assert grad_rgb.is_contiguous() and grad_rgb.dtype.is_floating_point
+        sc = scene.ex_struct()
+        cam = camera_struct(camera)
+        cfg = config.ex_struct(0)
+        g = grads.ex_struct()
+        gd = grad_depth.data_ptr() if grad_depth is not None else None
+        st = _lib.lib().lumen_ex_render_backward(self._ctx, ctypes.byref(sc), ctypes.byref(cam),
+                                                 ctypes.byref(cfg), grad_rgb.data_ptr(), gd,
+                                                 ctypes.byref(g), _stream_handle(scene.device))
+        _lib.check("lumen_ex_render_backward", st)
 
     def set_profiling(self, enable: bool) -> None:
         _lib.check("lumen_ex_set_profiling",

@@ -115,6 +115,29 @@ float orc_pixel_alpha(float px, float py, float cx, float cy, const float inv[3]
 int64_t orc_blend_sequence(const float *colors, const float *alphas,
                            const float *depths, int64_t n, float state[5]);
 
+/* ---- FP64 forward and analytic backward (lumen_oracle_bwd.c) ----------- */
+typedef struct {
+    int64_t n;
+    const double *planes; /* 11 x n, GaussianSoA plane order */
+    const double *sh;     /* n x 48 */
+} orc_scene64;
+
+/* GaussianGradients (src/optim/gradients.mojo:10-106), accumulated into. */
+typedef struct {
+    double *planes;       /* 11 x n: pos xyz, scale xyz, rot wxyz, opacity */
+    double *sh;           /* n x 48 */
+    double *viewspace;    /* n: sum over views of |dL/d mean2d| (pixels) */
+    int64_t *count;       /* n: views in which the Gaussian was visible */
+} orc_grads64;
+
+int orc64_forward(const orc_scene64 *sc, const orc_camera *cam, const orc_config *cfg,
+                  double *rgb, double *depth, uint64_t *signature);
+/* g_vis (optional, V x 10): per visible splat dL/d(mean x, mean y, inv00,
+ * inv01, inv11, opacity, r, g, b, depth).  Returns V. */
+int orc64_backward(const orc_scene64 *sc, const orc_camera *cam, const orc_config *cfg,
+                   const double *g_rgb, const double *g_depth, orc_grads64 *out,
+                   double *g_vis);
+
 /* ---- losses (src/optim/loss.mojo) -------------------------------------- */
 float orc_l1_loss(const float *pred, const float *target, int64_t n, float *grad);
 float orc_ssim(const float *pred, const float *target, int64_t w, int64_t h, int64_t win);

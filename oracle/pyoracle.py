@@ -216,3 +216,75 @@ def expf_array(x: np.ndarray) -> np.ndarray:
     y = np.empty_like(x)
     lib().orc_expf_array(fptr(x), fptr(y), x.size)
     return y
+
+
+# ---- FP64 forward / backward ---------------------------------------------
+f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcScene64(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("planes", f64p), ("sh", f64p)]
+
+
+class OrcGrads64(ctypes.Structure):
+    _fields_ = [("planes", f64p), ("sh", f64p), ("viewspace", f64p), ("count", i64p)]
+
+
+def _lib64():
+    L = lib()
+    if not getattr(L, "_has64", False):
+        L.orc64_forward.restype = ctypes.c_int
+        L.orc64_forward.argtypes = [ctypes.POINTER(OrcScene64), ctypes.POINTER(OrcCamera),
+                                    ctypes.POINTER(OrcConfig), f64p, f64p,
+                                    ctypes.POINTER(ctypes.c_uint64)]
+        L.orc64_backward.restype = ctypes.c_int
+        L.orc64_backward.argtypes = [ctypes.POINTER(OrcScene64), ctypes.POINTER(OrcCamera),
+                                     ctypes.POINTER(OrcConfig), f64p, f64p,
+                                     ctypes.POINTER(OrcGrads64), f64p]
+        L._has64 = True
+    return L
+
+
+def _scene64(planes, sh):
+    planes = np.ascontiguousarray(planes, np.float64)
+    sh = np.ascontiguousarray(sh, np.float64)
+    s = OrcScene64()
+    s.n = planes.shape[1]
+    s.planes = planes.ctypes.data_as(f64p)
+    s.sh = sh.ctypes.data_as(f64p)
+    return s, (planes, sh)
+
+
+def forward64(planes, sh, camera, config):
+    """FP64 render: (rgb (H,W,3), depth (H,W), decision signature)."""
+    L = _lib64()
+    s, keep = _scene64(planes, sh)
+    W, H = camera.config.width, camera.config.height
+    rgb = np.zeros((H, W, 3), np.float64)
+    depth = np.zeros((H, W), np.float64)
+    sig = ctypes.c_uint64(0)
+    L.orc64_forward(ctypes.byref(s), ctypes.byref(camera_struct(camera)),
+                    ctypes.byref(config_struct(config)), rgb.ctypes.data_as(f64p),
+                    depth.ctypes.data_as(f64p), ctypes.byref(sig))
+    return rgb, depth, int(sig.value)
+
+
+def backward64(planes, sh, camera, config, g_rgb, g_depth):
+    """Analytic gradient of sum(g_rgb * rgb) + sum(g_depth * depth)."""
+    L = _lib64()
+    s, keep = _scene64(planes, sh)
+    n = s.n
+    gp = np.zeros((11, n), np.float64)
+    gsh = np.zeros((n, 48), np.float64)
+    vs = np.zeros(n, np.float64)
+    cnt = np.zeros(n, np.int64)
+    gr = OrcGrads64(gp.ctypes.data_as(f64p), gsh.ctypes.data_as(f64p), vs.ctypes.data_as(f64p),
+                    cnt.ctypes.data_as(i64p))
+    g_rgb = np.ascontiguousarray(g_rgb, np.float64)
+    g_depth = np.ascontiguousarray(g_depth, np.float64)
+    gvis = np.zeros((max(n, 1), 10), np.float64)
+    V = L.orc64_backward(ctypes.byref(s), ctypes.byref(camera_struct(camera)),
+                         ctypes.byref(config_struct(config)), g_rgb.ctypes.data_as(f64p),
+                         g_depth.ctypes.data_as(f64p), ctypes.byref(gr),
+                         gvis.ctypes.data_as(f64p))
+    return {"planes": gp, "sh": gsh, "viewspace": vs, "count": cnt, "per_visible": gvis[:V]}
