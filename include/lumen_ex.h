@@ -102,6 +102,37 @@ LumenStatus lumen_ex_render(LumenExContext* ctx, const LumenExScene* scene,
                             float* out_rgb, float* out_depth, LumenExRenderStats* out_stats,
                             void* stream);
 
+/* ---- many views of one scene in one frame ---------------------------------
+ * Renders rasterize_scene for every camera with one launch per stage: view k
+ * owns the output pixels [pix_base_k, pix_base_k + W_k*H_k) of out_rgb
+ * (x3 floats) and out_depth, pix_base_k = sum of W*H over views < k.  Each
+ * view's image, and its keys/values/ranges with the view's frame offsets
+ * removed, equal a single-view lumen_ex_render of that camera.  All views
+ * share the config (tile shape, SH degree, ...); resolutions may differ. */
+#define LUMEN_EX_GRAPH 4u   /* capture the frame as a hipGraph on first use (needs a
+                               non-default stream) and replay it while the scene,
+                               cameras, config, outputs and stream are unchanged; no
+                               host round trip per frame */
+
+typedef struct {
+    uint32_t vis_base;   /* first visible index of the view in the frame's arrays */
+    uint32_t visible;    /* RenderOutput.visible_splats of the view */
+    uint32_t key_base;   /* first key of the view in the frame's (unsorted) key array */
+    uint32_t instances;  /* RenderOutput.total_splat_instances of the view */
+} LumenExViewStats;
+
+LumenStatus lumen_ex_render_batch(LumenExContext* ctx, const LumenExScene* scene,
+                                  const LumenExCamera* cameras, uint32_t n_views,
+                                  const LumenExRenderConfig* config, float* out_rgb,
+                                  float* out_depth, LumenExViewStats* out_stats, void* stream);
+
+/* Waits for the context's last frame on `stream` and copies its per-view
+ * stats (out_stats may be NULL).  Returns LUMEN_EX_ERROR_CAPACITY when a graph
+ * replay produced more tile instances than the capacity reserved when it was
+ * captured (the frame's images are then background only). */
+LumenStatus lumen_ex_frame_status(LumenExContext* ctx, LumenExViewStats* out_stats,
+                                  uint32_t n_views, void* stream);
+
 /* radix_sort_64bit (src/raster/radix_sort.mojo:39-87): stable ascending sort
  * of n 64-bit keys with 32-bit values, in place on device memory.  Digits
  * that are equal across all keys are skipped (a stable pass over them is the
@@ -169,6 +200,16 @@ LumenStatus lumen_ex_render_backward(LumenExContext* ctx, const LumenExScene* sc
                                      const LumenExRenderConfig* config, const float* grad_rgb,
                                      const float* grad_depth, LumenExGradients* grads,
                                      void* stream);
+
+/* Backward of the last lumen_ex_render_batch (same cameras and config, with
+ * LUMEN_EX_SAVE_PIXELS); grad_rgb/grad_depth in the batch's output layout.
+ * Works after a graph replay too.  Gradients of a Gaussian seen in several
+ * views are summed (atomically, so the summation order is not fixed). */
+LumenStatus lumen_ex_render_batch_backward(LumenExContext* ctx, const LumenExScene* scene,
+                                           const LumenExCamera* cameras, uint32_t n_views,
+                                           const LumenExRenderConfig* config,
+                                           const float* grad_rgb, const float* grad_depth,
+                                           LumenExGradients* grads, void* stream);
 
 /* Per-stage GPU timing with HIP events recorded on the render's stream around
  * each stage of lumen_ex_render (off by default).  lumen_ex_stage_times waits

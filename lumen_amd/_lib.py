@@ -85,6 +85,11 @@ class LumenExRenderStats(ctypes.Structure):
                 ("sort_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class LumenExViewStats(ctypes.Structure):
+    _fields_ = [("vis_base", ctypes.c_uint32), ("visible", ctypes.c_uint32),
+                ("key_base", ctypes.c_uint32), ("instances", ctypes.c_uint32)]
+
+
 class LumenExGradients(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x", "rot_y",
@@ -93,6 +98,8 @@ class LumenExGradients(ctypes.Structure):
 
 LUMEN_EX_KEEP_STAGES = 1
 LUMEN_EX_SAVE_PIXELS = 2
+LUMEN_EX_GRAPH = 4
+LUMEN_EX_ERROR_CAPACITY = -101
 
 STAGES = {"vis_to_orig": 0, "records": 1, "radius": 2, "keys_unsorted": 3, "vals_unsorted": 4,
           "keys": 5, "vals": 6, "ranges": 7, "t_final": 8, "n_eval": 9, "grad_vis": 10}
@@ -154,6 +161,21 @@ _SIGS = {
                                                 ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.POINTER(LumenExGradients),
                                                 ctypes.c_void_p]),
+    "lumen_ex_render_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenExScene),
+                                             ctypes.POINTER(LumenExCamera), ctypes.c_uint32,
+                                             ctypes.POINTER(LumenExRenderConfig), ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.POINTER(LumenExViewStats),
+                                             ctypes.c_void_p]),
+    "lumen_ex_frame_status": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenExViewStats),
+                                             ctypes.c_uint32, ctypes.c_void_p]),
+    "lumen_ex_render_batch_backward": (ctypes.c_int, [ctypes.c_void_p,
+                                                      ctypes.POINTER(LumenExScene),
+                                                      ctypes.POINTER(LumenExCamera),
+                                                      ctypes.c_uint32,
+                                                      ctypes.POINTER(LumenExRenderConfig),
+                                                      ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.POINTER(LumenExGradients),
+                                                      ctypes.c_void_p]),
     "lumen_ex_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lumen_ex_stage_times": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
 }

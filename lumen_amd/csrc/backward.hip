@@ -33,20 +33,46 @@ namespace lumen {
 
 constexpr int kGradVis = 10;  // mean x, mean y, inv00, inv01, inv11, opacity, r, g, b, depth
 
+__device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ views, int nviews,
+                                              uint32_t gt) {
+  int lo = 0, hi = nviews - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (views[mid].tile_base <= gt) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_zero_grad_vis(float* __restrict__ g, const SortPlan* __restrict__ plan) {
+  const uint64_t n = (uint64_t)plan->v * kGradVis;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    g[i] = 0.0f;
+}
+
 template <int BATCH>
 __global__ __launch_bounds__(256) void k_composite_bwd(
-    const int2* __restrict__ ranges, const uint32_t* __restrict__ vals,
-    const float4* __restrict__ rec, ViewParams v, const float* __restrict__ t_final,
+    const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
+    const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
+    const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
+    const ViewSpan* __restrict__ spans, const float* __restrict__ t_final,
     const int32_t* __restrict__ n_eval, const float* __restrict__ g_rgb,
     const float* __restrict__ g_depth, float* __restrict__ grad_vis) {
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint32_t s_vi[BATCH];
   __shared__ float s_acc[BATCH * kGradVis];
   __shared__ int s_last;
+  if (plan->overflow) return;
+  const uint32_t gt = blockIdx.x;
+  const int vw = view_of_tile_b(views, nviews, gt);
+  const ViewParams& v = views[vw];
+  const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int nthreads = v.tile_w * v.tile_h;
-  const int tile = blockIdx.x;
-  const int2 rg = ranges[tile];
+  const int tile = (int)(gt - v.tile_base);
+  const int2 rg = ranges[gt];
   if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
+  const bool white = v.white_background && spans[vw].instances > 0;
   const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
   const int t = threadIdx.x;
   const int px = tx * v.tile_w + t % v.tile_w;
@@ -58,7 +84,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   int last = rg.x - 1;
   float T = 1.0f, gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gD = 0.0f;
   if (inside) {
-    const size_t pix = (size_t)py * v.width + px;
+    const size_t pix = v.pix_base + (size_t)py * v.width + px;
     last = rg.x + n_eval[pix] - 1;
     T = t_final[pix];
     gC0 = g_rgb[3 * pix + 0];
@@ -122,7 +148,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
             gv[9] = gD * w;
             float dal = gC0 * (b.z * Tj - SC0 / om) + gC1 * (b.w * Tj - SC1 / om) +
                         gC2 * (c.x * Tj - SC2 / om) + gD * (c.y * Tj - SD / om);
-            if (v.white_background) dal -= gsum * Tf / om;
+            if (white) dal -= gsum * Tf / om;
             SC0 += b.z * w;
             SC1 += b.w * w;
             SC2 += c.x * w;
@@ -219,12 +245,25 @@ struct GradPlanes {
   int32_t* count;
 };
 
-__global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v, uint32_t V,
+// One view per frame: each Gaussian is written by one thread, plain adds.
+// Several views: a Gaussian visible in several views is written by several
+// threads, so the adds are atomic.
+__device__ __forceinline__ void acc(float* p, float x, bool atomic) {
+  if (atomic) atomicAdd(p, x);
+  else *p += x;
+}
+
+__global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
+                                                        const ViewParams* __restrict__ views,
+                                                        const SortPlan* __restrict__ plan,
+                                                        const float4* __restrict__ rec,
                                                         const int32_t* __restrict__ vis2orig,
                                                         const float* __restrict__ grad_vis,
-                                                        GradPlanes out) {
+                                                        GradPlanes out, int atomic_adds) {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  if (k >= V) return;
+  if (k >= plan->v || plan->overflow) return;
+  const bool atomic = atomic_adds != 0;
+  const ViewParams& v = views[__float_as_int(rec[3u * k + 2u].z)];
   const uint32_t i = (uint32_t)vis2orig[k];
   const float* g = grad_vis + (size_t)k * kGradVis;
   const float gmx = g[0], gmy = g[1], ga = g[2], gb = g[3], gc = g[4];
@@ -232,7 +271,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
   float gpos[3] = {0.0f, 0.0f, 0.0f};
 
   // opacity
-  out.p[10][i] += g[5];
+  acc(&out.p[10][i], g[5], atomic);
 
   // colour = clamp(0.5 + sum_k sh_k Y_k(dir)), dir = normalize(pos - eye)
   const float px = s.px[i], py = s.py[i], pz = s.pz[i];
@@ -253,7 +292,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
     dY[kk] = 0.0f;
     if (kk >= v.sh_coeffs) continue;
     for (int ch = 0; ch < 3; ++ch) {
-      gsh[3 * kk + ch] += gcol[ch] * Y[kk];
+      acc(&gsh[3 * kk + ch], gcol[ch] * Y[kk], atomic);
       dY[kk] += gcol[ch] * sh[3 * kk + ch];
     }
   }
@@ -276,8 +315,11 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
   gpc[0] = gmx * fx / z;
   gpc[1] = gmy * fy / z;
   gpc[2] = g[9] - gmx * fx * x / (z * z) - gmy * fy * y / (z * z);
-  if (out.viewspace) out.viewspace[i] += sqrtf(gmx * gmx + gmy * gmy);
-  if (out.count) out.count[i] += 1;
+  if (out.viewspace) acc(&out.viewspace[i], sqrtf(gmx * gmx + gmy * gmy), atomic);
+  if (out.count) {
+    if (atomic) atomicAdd(&out.count[i], 1);
+    else out.count[i] += 1;
+  }
 
   // world covariance Sigma = M M^T, M = R(q) diag(s)
   const Quat qraw{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
@@ -341,9 +383,9 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
   gpc[0] += gj02 * (-fx * itz2);
   gpc[1] += gj12 * (-fy * itz2);
   for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
-  out.p[0][i] += gpos[0];
-  out.p[1][i] += gpos[1];
-  out.p[2][i] += gpos[2];
+  acc(&out.p[0][i], gpos[0], atomic);
+  acc(&out.p[1][i], gpos[1], atomic);
+  acc(&out.p[2][i], gpos[2], atomic);
 
   // Sigma -> scale and rotation
   float gM[9], gR[9];
@@ -352,7 +394,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
       gM[3 * rr + cc] = 2.0f * (F[rr][0] * M[cc] + F[rr][1] * M[3 + cc] + F[rr][2] * M[6 + cc]);
   for (int cc = 0; cc < 3; ++cc) {
     const float gs = gM[cc] * R[cc] + gM[3 + cc] * R[3 + cc] + gM[6 + cc] * R[6 + cc];
-    if (!sclamp[cc]) out.p[3 + cc][i] += gs;
+    if (!sclamp[cc]) acc(&out.p[3 + cc][i], gs, atomic);
   }
   for (int rr = 0; rr < 3; ++rr)
     for (int cc = 0; cc < 3; ++cc) gR[3 * rr + cc] = gM[3 * rr + cc] * sc[cc];
@@ -368,7 +410,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
   if (qn > 0.000001f) {
     const float dot = gq[0] * w + gq[1] * qx + gq[2] * qy + gq[3] * qz;
     const float qq[4] = {w, qx, qy, qz};
-    for (int a = 0; a < 4; ++a) out.p[6 + a][i] += (gq[a] - qq[a] * dot) / qn;
+    for (int a = 0; a < 4; ++a) acc(&out.p[6 + a][i], (gq[a] - qq[a] * dot) / qn, atomic);
   }
 }
 
@@ -376,21 +418,24 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s, ViewParams v
 
 using namespace lumen;
 
-extern "C" LumenStatus lumen_ex_render_backward(LumenExContext* c, const LumenExScene* sc,
-                                                const LumenExCamera* cam,
-                                                const LumenExRenderConfig* cfg,
-                                                const float* grad_rgb, const float* grad_depth,
-                                                LumenExGradients* grads, void* stream) {
-  clear_error();
-  if (!c || !sc || !cam || !cfg || !grad_rgb || !grads)
+static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
+                                 const LumenExCamera* cams, uint32_t n_views,
+                                 const LumenExRenderConfig* cfg, const float* grad_rgb,
+                                 const float* grad_depth, LumenExGradients* grads, void* stream) {
+  if (!c || !sc || !cams || !cfg || !grad_rgb || !grads)
     return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_backward: null argument");
   if (!c->pixels)
     return fail(LUMEN_ERROR_UNINITIALIZED,
-                "lumen_ex_render_backward: the last render did not keep LUMEN_EX_SAVE_PIXELS");
-  if (sc->count != c->N || cam->width != c->width || cam->height != c->height ||
-      cfg->tile_width != c->tile_w || cfg->tile_height != c->tile_h)
+                "lumen_ex_render_backward: the last frame did not keep LUMEN_EX_SAVE_PIXELS");
+  if (sc->count != c->N || n_views != c->nviews || cfg->tile_width != c->tile_w ||
+      cfg->tile_height != c->tile_h)
     return fail(LUMEN_ERROR_INVALID_ARGUMENT,
-                "lumen_ex_render_backward: scene/camera/config differ from the last render");
+                "lumen_ex_render_backward: scene/cameras/config differ from the last frame");
+  for (uint32_t k = 0; k < n_views; ++k)
+    if ((int32_t)cams[k].width != c->host_views[k].width ||
+        (int32_t)cams[k].height != c->host_views[k].height)
+      return fail(LUMEN_ERROR_INVALID_ARGUMENT,
+                  "lumen_ex_render_backward: view %u differs from the last frame", k);
   float* planes[11] = {grads->pos_x, grads->pos_y, grads->pos_z, grads->scale_x, grads->scale_y,
                        grads->scale_z, grads->rot_w, grads->rot_x, grads->rot_y, grads->rot_z,
                        grads->opacity};
@@ -399,34 +444,42 @@ extern "C" LumenStatus lumen_ex_render_backward(LumenExContext* c, const LumenEx
   if (!grads->sh) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_backward: null sh gradient");
   hipStream_t st = (hipStream_t)stream;
   LUMEN_HIP(hipSetDevice(c->device));
-  const uint64_t V = c->V;
   c->has_grad = true;
-  if (V == 0 || c->M == 0) return LUMEN_SUCCESS;
-  ViewParams v;
-  fill_view(v, cam, cfg);
-  LUMEN_TRY(ensure(c->grad_vis, V * kGradVis * sizeof(float)));
-  LUMEN_TRY(fill<float>(c->grad_vis.as<float>(), 0.0f, V * kGradVis, st));
+  // Sizes: exact after a plain frame, the reserved capacity after a graph
+  // replay (the kernels read the frame's own counts from the device plan).
+  const uint64_t v_cap = c->sizes_known ? c->V : c->v_cap;
+  if (sc->count == 0 || v_cap == 0 || (c->sizes_known && c->M == 0)) return LUMEN_SUCCESS;
+  LUMEN_TRY(ensure(c->grad_vis, v_cap * kGradVis * sizeof(float)));
+  const SortPlan* plan = c->plan.as<SortPlan>();
+  float* gv = c->grad_vis.as<float>();
+  const uint32_t zb = div_up(v_cap * kGradVis, 256) < 4096 ? div_up(v_cap * kGradVis, 256) : 4096;
+  hipLaunchKernelGGL(k_zero_grad_vis, dim3(zb), dim3(256), 0, st, gv, plan);
+  LUMEN_LAUNCHED();
   const uint32_t threads = cfg->tile_width * cfg->tile_height;
   const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
   const int2* ranges = c->ranges.as<int2>();
-  const uint32_t* vals = c->vals[c->final_buf].as<uint32_t>();
+  const uint32_t* v0 = c->vals[0].as<uint32_t>();
+  const uint32_t* v1 = c->vals[1].as<uint32_t>();
   const float4* rec = c->rec.as<float4>();
+  const ViewParams* views = c->views.as<ViewParams>();
+  const ViewSpan* spans = c->spans.as<ViewSpan>();
   const float* tf = c->t_final.as<float>();
   const int32_t* ne = c->n_eval.as<int32_t>();
-  float* gv = c->grad_vis.as<float>();
+  const int nv = (int)c->nviews;
+  const dim3 grid((uint32_t)c->T), block(threads);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
   switch (batch) {
     case 64:
-      hipLaunchKernelGGL(k_composite_bwd<64>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
-                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      hipLaunchKernelGGL(k_composite_bwd<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
       break;
     case 128:
-      hipLaunchKernelGGL(k_composite_bwd<128>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
-                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      hipLaunchKernelGGL(k_composite_bwd<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
       break;
     default:
-      hipLaunchKernelGGL(k_composite_bwd<256>, dim3((uint32_t)c->T), dim3(threads), 0, st, ranges,
-                         vals, rec, v, tf, ne, grad_rgb, grad_depth, gv);
+      hipLaunchKernelGGL(k_composite_bwd<256>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
       break;
   }
   LUMEN_LAUNCHED();
@@ -437,9 +490,28 @@ extern "C" LumenStatus lumen_ex_render_backward(LumenExContext* c, const LumenEx
   gp.sh = grads->sh;
   gp.viewspace = grads->viewspace_grad_2d;
   gp.count = grads->observation_count;
-  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(V, 256)), dim3(256), 0, st, scene_dev(sc), v,
-                     (uint32_t)V, c->vis2orig.as<int32_t>(), gv, gp);
+  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(v_cap, 256)), dim3(256), 0, st, scene_dev(sc),
+                     views, plan, rec, c->vis2orig.as<int32_t>(), gv, gp, nv > 1 ? 1 : 0);
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, true, st));
   return LUMEN_SUCCESS;
+}
+
+extern "C" LumenStatus lumen_ex_render_backward(LumenExContext* c, const LumenExScene* sc,
+                                                const LumenExCamera* cam,
+                                                const LumenExRenderConfig* cfg,
+                                                const float* grad_rgb, const float* grad_depth,
+                                                LumenExGradients* grads, void* stream) {
+  clear_error();
+  return backward_impl(c, sc, cam, 1, cfg, grad_rgb, grad_depth, grads, stream);
+}
+
+extern "C" LumenStatus lumen_ex_render_batch_backward(LumenExContext* c, const LumenExScene* sc,
+                                                      const LumenExCamera* cams, uint32_t n_views,
+                                                      const LumenExRenderConfig* cfg,
+                                                      const float* grad_rgb,
+                                                      const float* grad_depth,
+                                                      LumenExGradients* grads, void* stream) {
+  clear_error();
+  return backward_impl(c, sc, cams, n_views, cfg, grad_rgb, grad_depth, grads, stream);
 }

@@ -1,9 +1,14 @@
 // context.hpp -- the lumen_ex context (per-frame workspace and the state one
-// render leaves for its backward pass) and the device view of a scene.
+// frame leaves for its backward pass) and the device view of a scene.
 #pragma once
+
+#include <string.h>
+
+#include <vector>
 
 #include "common.hpp"
 #include "lumen_ex.h"
+#include "plan.hpp"
 #include "splat_math.hpp"
 
 namespace lumen {
@@ -48,22 +53,42 @@ inline SceneDev scene_dev(const LumenExScene* sc) {
                   (uint32_t)sc->count};
 }
 
+// Where each view's visible splats and keys sit in the frame's arrays.
+struct ViewSpan {
+  uint32_t vis_base, visible, key_base, instances;
+};
+
 }  // namespace lumen
+
+struct GraphCache {
+  bool valid = false;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<uint8_t> key;  // bytes of everything the captured frame depends on
+  uint64_t m_cap = 0;
+};
 
 struct LumenExContext {
   int device = 0;
-  lumen::DevBuf packed, part, totals, bounds;
+  lumen::DevBuf packed, part, totals, bounds, plan, views, spans;
   lumen::DevBuf rec, vis2orig, radius;
   lumen::DevBuf keys[2], vals[2], keys_keep, vals_keep;
   lumen::DevBuf hist, hist_part;
   lumen::DevBuf ranges, t_final, n_eval, minmax;
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
-  void* pinned = nullptr;
-  // metadata of the last render (for lumen_ex_copy_stage and the backward)
+  void* pinned = nullptr;       // small host staging (totals, bounds, plan)
+  void* pinned_views = nullptr; // host staging of the per-view parameters
+  size_t pinned_views_cap = 0;
+  // the last frame (for lumen_ex_copy_stage and the backward)
+  std::vector<lumen::ViewParams> host_views;
   uint64_t V = 0, M = 0, T = 0, P = 0, N = 0;
-  uint32_t width = 0, height = 0, tile_w = 0, tile_h = 0;
-  int final_buf = 0;
+  uint32_t nviews = 0;
+  uint32_t tile_w = 0, tile_h = 0;
+  uint64_t v_cap = 0, m_cap = 0;  // capacities the frame's arrays were sized for
+  bool sizes_known = false;        // V and M were read back (not a graph replay)
   bool kept = false, pixels = false, has_grad = false;
+  GraphCache graph;
   // optional per-stage timing (lumen_ex_set_profiling)
   bool profile = false;
   bool events_ready = false;
@@ -71,11 +96,13 @@ struct LumenExContext {
   hipEvent_t ev[2 * LUMEN_EX_TIME_STAGES] = {};
 };
 
-
 // Records the start (end = false) or end of a timed stage on the stream when
-// profiling is enabled (lumen_ex_set_profiling).
+// profiling is enabled (lumen_ex_set_profiling).  Never inside a graph capture.
 inline LumenStatus mark(LumenExContext* c, int stage, bool end, hipStream_t st) {
   if (!c->profile) return LUMEN_SUCCESS;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  LUMEN_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return LUMEN_SUCCESS;
   LUMEN_HIP(hipEventRecord(c->ev[2 * stage + (end ? 1 : 0)], st));
   if (end) c->stage_ran[stage] = true;
   return LUMEN_SUCCESS;

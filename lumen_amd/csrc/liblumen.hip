@@ -1,19 +1,29 @@
 // liblumen.hip -- forward splat rasterization on MI355X (gfx950) behind the
-// lumen_ex.h C ABI.  The stages and the reference code they replace:
+// lumen_ex.h C ABI.  A frame renders one scene from one or more views; each
+// view is exactly rasterize_scene (src/raster/pipeline.mojo:48-231):
 //
 //   k_count      culling + covariance + EWA + 3-sigma radius + tile rectangle
-//                per Gaussian; writes (tile count << 32 | visible) for the
-//                compaction scan      src/raster/pipeline.mojo:84-130
-//   scan         visible index and key offset per Gaussian (stable, so the
-//                visible order is the reference's append order) :141-142
-//   k_emit       same geometry again (bit-identical), SH colour, inverse
-//                covariance, the 48-byte splat record at its visible index and
-//                the 64-bit tile|depth keys      :132-157
-//   sort         stable LSD radix sort over the varying digits    :166
-//   k_ranges     per-tile [start, end)                            :169
-//   k_composite  one workgroup per tile, one lane per pixel, splat records
-//                staged through LDS in batches, per-pixel front-to-back blend
-//                with T <= 1e-4 termination and a block-wide exit :172-229
+//                per (view, Gaussian), one thread per Gaussian looping over
+//                the views; writes (tile count << 32 | visible) for the
+//                compaction scan (view-major)       pipeline.mojo:84-130
+//   scan         visible index and key offset per (view, Gaussian); stable,
+//                so each view's visible order is the reference's append
+//                order                               :141-142
+//   k_plan       sort digit plan, key count and per-view spans, on the device
+//   k_emit       the same geometry (bit-identical), SH colour, inverse
+//                covariance, the 48-byte splat record at its visible index,
+//                and the 64-bit keys (frame tile id << 32 | depth)  :132-157
+//   sort         stable LSD radix sort over the digits that vary    :166
+//   k_ranges     per-tile [start, end)                               :169
+//   k_composite  one workgroup per tile, one lane per pixel, records staged
+//                through LDS in batches, per-pixel front-to-back blend with
+//                T <= 1e-4 termination and a block-wide exit         :172-229
+//
+// A view's tiles are frame tiles [tile_base, tile_base + tiles); with one
+// view the frame tile id is the reference's tile id, so keys, values and
+// ranges are the reference's.  The plain path blocks the host once per frame
+// (to size the key buffers); the graph path (LUMEN_EX_GRAPH) captures the
+// whole frame once with reserved capacities and replays it.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -21,11 +31,11 @@
 #include <string>
 
 #include "common.hpp"
+#include "context.hpp"
 #include "lumen_ex.h"
 #include "scan.hpp"
 #include "sort.hpp"
 #include "splat_math.hpp"
-#include "context.hpp"
 
 namespace lumen {
 
@@ -42,11 +52,15 @@ void set_error(const char* fmt, ...) {
 const char* last_error() { return t_last_error.c_str(); }
 void clear_error() { t_last_error.clear(); }
 
-__global__ void k_init_bounds(uint32_t* b) {
-  b[0] = 0xFFFFFFFFu;
-  b[1] = 0u;
-  b[2] = 0xFFFFFFFFu;
-  b[3] = 0u;
+__global__ void k_init_frame(uint32_t* __restrict__ bounds, SortPlan* __restrict__ plan) {
+  bounds[0] = 0xFFFFFFFFu;
+  bounds[1] = 0u;
+  bounds[2] = 0xFFFFFFFFu;
+  bounds[3] = 0u;
+  plan->passes = 0;
+  plan->m = 0;
+  plan->v = 0;
+  plan->overflow = 0;
 }
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
@@ -60,26 +74,35 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
   return x;
 }
 
-// Pass 1: visibility and tile count per Gaussian, plus the depth-key and
-// tile-id bounds that decide which sort digits can vary.
-__global__ __launch_bounds__(256) void k_count(SceneDev s, ViewParams v,
-                                               uint64_t* __restrict__ packed,
+// Pass 1: visibility and tile count per (view, Gaussian), plus the depth-key
+// and frame-tile-id bounds that decide which sort digits can vary.
+__global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
+                                               int nviews, uint64_t* __restrict__ packed,
                                                uint32_t* __restrict__ bounds) {
   __shared__ uint32_t sb[4][4];
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
   if (i < s.n) {
-    SplatGeom g;
-    uint64_t val = 0;
-    if (load_geometry(s, v, i, g)) {
-      const uint32_t cnt =
-          (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
-      val = ((uint64_t)cnt << 32) | 1ull;
-      dmin = dmax = depth_key(g.depth);
-      tmin = (uint32_t)(g.min_ty * v.tiles_x + g.min_tx);
-      tmax = (uint32_t)(g.max_ty * v.tiles_x + g.max_tx);
+    const float px = s.px[i], py = s.py[i], pz = s.pz[i];
+    const float s0 = s.sx[i], s1 = s.sy[i], s2 = s.sz[i];
+    const Quat q{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
+    const float op = s.op[i];
+    for (int vw = 0; vw < nviews; ++vw) {
+      const ViewParams& v = views[vw];
+      SplatGeom g;
+      uint64_t val = 0;
+      if (splat_geometry(v, px, py, pz, s0, s1, s2, q, op, g)) {
+        const uint32_t cnt =
+            (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
+        val = ((uint64_t)cnt << 32) | 1ull;
+        const uint32_t dk = depth_key(g.depth);
+        dmin = min(dmin, dk);
+        dmax = max(dmax, dk);
+        tmin = min(tmin, v.tile_base + (uint32_t)(g.min_ty * v.tiles_x + g.min_tx));
+        tmax = max(tmax, v.tile_base + (uint32_t)(g.max_ty * v.tiles_x + g.max_tx));
+      }
+      packed[(size_t)vw * s.n + i] = val;
     }
-    packed[i] = val;
   }
   dmin = wave_min_u32(dmin);
   dmax = wave_max_u32(dmax);
@@ -109,78 +132,156 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, ViewParams v,
   }
 }
 
-// Pass 2: records and keys of visible Gaussians at their scanned offsets.
-__global__ __launch_bounds__(256) void k_emit(SceneDev s, ViewParams v,
-                                              const uint64_t* __restrict__ offsets,
+// Sort plan, clamped sizes and per-view spans from the scan and the bounds.
+__global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
+                       const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
+                       uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
+                       ViewSpan* __restrict__ spans) {
+  const uint64_t tot = *totals;
+  const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
+  if (threadIdx.x == 0) {
+    int32_t dg[8];
+    int np = 0;
+    if (V > 0) {
+      np = varying_digits(bounds[0], bounds[1], 0, 4, dg);
+      np += varying_digits(bounds[2], bounds[3], 4, 4, dg + np);
+    }
+    plan->passes = np;
+    for (int k = 0; k < 8; ++k) plan->digit[k] = k < np ? dg[k] : 0;
+    plan->m = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
+    plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
+    if ((uint64_t)M > m_cap || (uint64_t)V > v_cap) plan->overflow = 1u;
+  }
+  for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
+    const uint64_t a = offsets[(size_t)vw * n];
+    const uint64_t b = vw + 1 < nviews ? offsets[(size_t)(vw + 1) * n] : tot;
+    ViewSpan sp;
+    sp.vis_base = (uint32_t)a;
+    sp.visible = (uint32_t)b - (uint32_t)a;
+    sp.key_base = (uint32_t)(a >> 32);
+    sp.instances = (uint32_t)(b >> 32) - (uint32_t)(a >> 32);
+    spans[vw] = sp;
+  }
+}
+
+// Pass 2: records and keys of every visible (view, Gaussian) at its scanned
+// offsets.  The SH block is read once per Gaussian, for the first view that
+// sees it.
+__global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __restrict__ views,
+                                              int nviews, const uint64_t* __restrict__ offsets,
+                                              uint64_t m_cap, uint64_t v_cap,
                                               float4* __restrict__ rec,
                                               int32_t* __restrict__ vis2orig,
                                               int32_t* __restrict__ radius,
                                               uint64_t* __restrict__ keys,
-                                              uint32_t* __restrict__ vals) {
+                                              uint32_t* __restrict__ vals,
+                                              SortPlan* __restrict__ plan) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= s.n) return;
-  SplatGeom g;
-  if (!load_geometry(s, v, i, g)) return;
-  const uint64_t off = offsets[i];
-  const uint32_t vi = (uint32_t)off;
-  uint32_t ko = (uint32_t)(off >> 32);
-
-  // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
-  // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139
-  float dx = s.px[i] - v.eye[0], dy = s.py[i] - v.eye[1], dz = s.pz[i] - v.eye[2];
-  normalize3(dx, dy, dz);
-  float Y[16];
-  sh_basis(dx, dy, dz, Y);
-  const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+  const float px = s.px[i], py = s.py[i], pz = s.pz[i];
+  const float s0 = s.sx[i], s1 = s.sy[i], s2 = s.sz[i];
+  const Quat q{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
+  const float op = s.op[i];
+  bool have_sh = false;
   float c[48];
-  const int nfloat = 3 * v.sh_coeffs;
-#pragma unroll
-  for (int q = 0; q < 12; ++q) {
-    float4 x = (4 * q < nfloat) ? sh4[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-    c[4 * q + 0] = x.x;
-    c[4 * q + 1] = x.y;
-    c[4 * q + 2] = x.z;
-    c[4 * q + 3] = x.w;
-  }
-  float cr = 0.5f, cg = 0.5f, cb = 0.5f;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    if (k < v.sh_coeffs) {
-      cr += c[3 * k + 0] * Y[k];
-      cg += c[3 * k + 1] * Y[k];
-      cb += c[3 * k + 2] * Y[k];
+  for (int vw = 0; vw < nviews; ++vw) {
+    const ViewParams& v = views[vw];
+    SplatGeom g;
+    if (!splat_geometry(v, px, py, pz, s0, s1, s2, q, op, g)) continue;
+    const uint64_t off = offsets[(size_t)vw * s.n + i];
+    const uint32_t vi = (uint32_t)off;
+    uint32_t ko = (uint32_t)(off >> 32);
+    const uint32_t cnt = (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
+    if ((uint64_t)ko + cnt > m_cap || (uint64_t)vi >= v_cap) {
+      atomicOr(&plan->overflow, 1u);
+      continue;
     }
-  }
-  if (cr < 0.0f) cr = 0.0f; else if (cr > 1.0f) cr = 1.0f;
-  if (cg < 0.0f) cg = 0.0f; else if (cg > 1.0f) cg = 1.0f;
-  if (cb < 0.0f) cb = 0.0f; else if (cb > 1.0f) cb = 1.0f;
-
-  float i00, i01, i11;
-  inverse2d(g.c00, g.c01, g.c11, i00, i01, i11);
-  rec[3u * vi + 0u] = make_float4(g.sx, g.sy, i00, 2.0f * i01);
-  rec[3u * vi + 1u] = make_float4(i11, s.op[i], cr, cg);
-  rec[3u * vi + 2u] = make_float4(cb, g.depth, 0.0f, 0.0f);
-  vis2orig[vi] = (int32_t)i;
-  radius[vi] = g.radius;
-
-  // Emission order: ty-major then tx (pipeline.mojo:152-157).
-  const uint64_t dk = depth_key(g.depth);
-  for (int ty = g.min_ty; ty <= g.max_ty; ++ty) {
-    for (int tx = g.min_tx; tx <= g.max_tx; ++tx) {
-      keys[ko] = ((uint64_t)(uint32_t)(ty * v.tiles_x + tx) << 32) | dk;
-      vals[ko] = vi;
-      ++ko;
+    if (!have_sh) {
+      const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+      const int nfloat = 3 * v.sh_coeffs;
+#pragma unroll
+      for (int qd = 0; qd < 12; ++qd) {
+        const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+        c[4 * qd + 0] = x.x;
+        c[4 * qd + 1] = x.y;
+        c[4 * qd + 2] = x.z;
+        c[4 * qd + 3] = x.w;
+      }
+      have_sh = true;
+    }
+    // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
+    // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139
+    float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
+    normalize3(dx, dy, dz);
+    float Y[16];
+    sh_basis(dx, dy, dz, Y);
+    float cr = 0.5f, cg = 0.5f, cb = 0.5f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k < v.sh_coeffs) {
+        cr += c[3 * k + 0] * Y[k];
+        cg += c[3 * k + 1] * Y[k];
+        cb += c[3 * k + 2] * Y[k];
+      }
+    }
+    if (cr < 0.0f) cr = 0.0f; else if (cr > 1.0f) cr = 1.0f;
+    if (cg < 0.0f) cg = 0.0f; else if (cg > 1.0f) cg = 1.0f;
+    if (cb < 0.0f) cb = 0.0f; else if (cb > 1.0f) cb = 1.0f;
+    float i00, i01, i11;
+    inverse2d(g.c00, g.c01, g.c11, i00, i01, i11);
+    rec[3u * vi + 0u] = make_float4(g.sx, g.sy, i00, 2.0f * i01);
+    rec[3u * vi + 1u] = make_float4(i11, op, cr, cg);
+    rec[3u * vi + 2u] = make_float4(cb, g.depth, __int_as_float((int)vw), 0.0f);
+    vis2orig[vi] = (int32_t)i;
+    radius[vi] = g.radius;
+    // Emission order: ty-major then tx (pipeline.mojo:152-157).
+    const uint64_t dk = depth_key(g.depth);
+    for (int ty = g.min_ty; ty <= g.max_ty; ++ty) {
+      for (int tx = g.min_tx; tx <= g.max_tx; ++tx) {
+        keys[ko] = ((uint64_t)(v.tile_base + (uint32_t)(ty * v.tiles_x + tx)) << 32) | dk;
+        vals[ko] = vi;
+        ++ko;
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ keys, uint32_t m,
-                                                int2* __restrict__ ranges) {
+__global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
+                                                const uint64_t* __restrict__ k1,
+                                                const SortPlan* __restrict__ plan,
+                                                int2* __restrict__ ranges, uint32_t tiles) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t m = plan->m;
+  if (i >= m || plan->overflow) return;
+  const uint64_t* __restrict__ keys = (plan->passes & 1) ? k1 : k0;
+  const uint32_t tile = (uint32_t)(keys[i] >> 32);
+  if (tile >= tiles) return;
+  if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) ranges[tile].x = (int32_t)i;
+  if (i == m - 1 || (uint32_t)(keys[i + 1] >> 32) != tile) ranges[tile].y = (int32_t)(i + 1);
+}
+
+// Ranges of an arbitrary sorted key array (lumen_ex_build_ranges).
+__global__ __launch_bounds__(256) void k_ranges_plain(const uint64_t* __restrict__ keys,
+                                                      uint32_t m, int2* __restrict__ ranges,
+                                                      uint32_t tiles) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= m) return;
   const uint32_t tile = (uint32_t)(keys[i] >> 32);
+  if (tile >= tiles) return;
   if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) ranges[tile].x = (int32_t)i;
   if (i == m - 1 || (uint32_t)(keys[i + 1] >> 32) != tile) ranges[tile].y = (int32_t)(i + 1);
+}
+
+// The view owning frame tile gt (views are in tile_base order).
+__device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views, int nviews,
+                                            uint32_t gt) {
+  int lo = 0, hi = nviews - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (views[mid].tile_base <= gt) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // One workgroup per tile, one lane per pixel (lane t -> (t % tile_w, t /
@@ -190,22 +291,25 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ key
 // (src/raster/compositor.mojo:25-89).  A lane stops at T <= 1e-4; the block
 // stops when all its lanes have.
 template <int BATCH>
-__global__ __launch_bounds__(256) void k_composite(const int2* __restrict__ ranges,
-                                                   const uint32_t* __restrict__ vals,
-                                                   const float4* __restrict__ rec, ViewParams v,
-                                                   float* __restrict__ rgb,
-                                                   float* __restrict__ depth,
-                                                   float* __restrict__ t_final,
-                                                   int32_t* __restrict__ n_eval) {
+__global__ __launch_bounds__(256) void k_composite(
+    const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
+    const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
+    const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
+    const ViewSpan* __restrict__ spans, float* __restrict__ rgb, float* __restrict__ depth,
+    float* __restrict__ t_final, int32_t* __restrict__ n_eval) {
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+  const uint32_t gt = blockIdx.x;
+  const int vw = view_of_tile(views, nviews, gt);
+  const ViewParams& v = views[vw];
   const int nthreads = v.tile_w * v.tile_h;
-  const int tile = blockIdx.x;
+  const int tile = (int)(gt - v.tile_base);
   const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
   const int t = threadIdx.x;
   const int px = tx * v.tile_w + t % v.tile_w;
   const int py = ty * v.tile_h + t / v.tile_w;
   const bool inside = px < v.width && py < v.height;
-  const int2 rg = ranges[tile];
+  const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
+  const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f, D = 0.0f;
   int32_t ne = 0;
   if (rg.x != -1 && rg.x < rg.y) {
@@ -253,9 +357,12 @@ __global__ __launch_bounds__(256) void k_composite(const int2* __restrict__ rang
     }
   }
   if (!inside) return;
-  const size_t pix = (size_t)py * v.width + px;
-  // PixelAccumulator.finalize + RenderOutput writes, pipeline.mojo:182-229
-  const float bg = v.white_background ? T : 0.0f;
+  const size_t pix = v.pix_base + (size_t)py * v.width + px;
+  // PixelAccumulator.finalize + RenderOutput writes, pipeline.mojo:182-229.
+  // rasterize_scene returns its zero-filled output before any background fill
+  // when the view has no tile instance (pipeline.mojo:63-65, 162-163).
+  const bool white = v.white_background && spans[vw].instances > 0;
+  const float bg = white ? T : 0.0f;
   rgb[3 * pix + 0] = C0 + bg;
   rgb[3 * pix + 1] = C1 + bg;
   rgb[3 * pix + 2] = C2 + bg;
@@ -286,96 +393,263 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
   }
 }
 
+__global__ void k_set_plan(SortPlan* plan, int passes, const int32_t* digits, uint32_t m) {
+  plan->passes = passes;
+  for (int k = 0; k < 8; ++k) plan->digit[k] = k < passes ? digits[k] : 0;
+  plan->m = m;
+  plan->v = 0;
+  plan->overflow = 0;
+}
+
 }  // namespace lumen
 
 using namespace lumen;
 
+// ---- validation -------------------------------------------------------------
+
 static bool plane_ok(const float* p) { return p != nullptr; }
 
-static LumenStatus composite_launch(uint32_t batch, uint32_t tiles, uint32_t threads,
-                                    hipStream_t st, const int2* ranges, const uint32_t* vals,
-                                    const float4* rec, const ViewParams& v, float* rgb,
-                                    float* depth, float* t_final, int32_t* n_eval) {
-  switch (batch) {
-    case 64:
-      hipLaunchKernelGGL(k_composite<64>, dim3(tiles), dim3(threads), 0, st, ranges, vals, rec, v,
-                         rgb, depth, t_final, n_eval);
-      break;
-    case 128:
-      hipLaunchKernelGGL(k_composite<128>, dim3(tiles), dim3(threads), 0, st, ranges, vals, rec, v,
-                         rgb, depth, t_final, n_eval);
-      break;
-    default:
-      hipLaunchKernelGGL(k_composite<256>, dim3(tiles), dim3(threads), 0, st, ranges, vals, rec, v,
-                         rgb, depth, t_final, n_eval);
-      break;
-  }
-  LUMEN_LAUNCHED();
-  return LUMEN_SUCCESS;
-}
-
-static LumenStatus render_impl(LumenExContext* c, const LumenExScene* sc, const LumenExCamera* cam,
-                               const LumenExRenderConfig* cfg, float* out_rgb, float* out_depth,
-                               LumenExRenderStats* stats, hipStream_t st) {
-  if (!c || !sc || !cam || !cfg || !out_rgb)
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: null argument");
-  if (cam->width == 0 || cam->height == 0 || cam->width > 16384 || cam->height > 16384)
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: bad image size %ux%u",
-                cam->width, cam->height);
+static LumenStatus check_config(const LumenExRenderConfig* cfg) {
   const uint32_t tw = cfg->tile_width, th = cfg->tile_height;
   const uint32_t threads = tw * th;
   if (tw == 0 || th == 0 || threads < 64 || threads > 256 || threads % 64 != 0)
     return fail(LUMEN_ERROR_INVALID_ARGUMENT,
-                "lumen_ex_render: tile %ux%u unsupported (pixels per tile must be 64, 128, 192 "
-                "or 256)",
-                tw, th);
+                "tile %ux%u unsupported (pixels per tile must be 64, 128, 192 or 256)", tw, th);
   const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
   if (batch != 64 && batch != 128 && batch != 256)
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: batch_depth %u not in {64,128,256}",
-                batch);
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "batch_depth %u not in {64,128,256}", batch);
   if (cfg->max_sh_degree < 0 || cfg->max_sh_degree > 3)
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: max_sh_degree %d not in 0..3",
-                cfg->max_sh_degree);
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "max_sh_degree %d not in 0..3", cfg->max_sh_degree);
+  return LUMEN_SUCCESS;
+}
+
+static LumenStatus check_scene(const LumenExScene* sc) {
   if (sc->count >= 0x7FFFFFFFull)
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: %llu Gaussians exceed 2^31-1",
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "%llu Gaussians exceed 2^31-1",
                 (unsigned long long)sc->count);
   if (sc->count > 0 &&
       !(plane_ok(sc->pos_x) && plane_ok(sc->pos_y) && plane_ok(sc->pos_z) &&
         plane_ok(sc->scale_x) && plane_ok(sc->scale_y) && plane_ok(sc->scale_z) &&
         plane_ok(sc->rot_w) && plane_ok(sc->rot_x) && plane_ok(sc->rot_y) && plane_ok(sc->rot_z) &&
         plane_ok(sc->opacity) && plane_ok(sc->sh)))
-    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: null scene plane");
-  LUMEN_HIP(hipSetDevice(c->device));
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "null scene plane");
+  return LUMEN_SUCCESS;
+}
 
-  ViewParams v;
-  fill_view(v, cam, cfg);
+static LumenStatus check_cameras(const LumenExCamera* cams, uint32_t n) {
+  if (n == 0 || n > 4096) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "%u views (1..4096)", n);
+  for (uint32_t k = 0; k < n; ++k)
+    if (cams[k].width == 0 || cams[k].height == 0 || cams[k].width > 16384 ||
+        cams[k].height > 16384)
+      return fail(LUMEN_ERROR_INVALID_ARGUMENT, "view %u: bad image size %ux%u", k, cams[k].width,
+                  cams[k].height);
+  return LUMEN_SUCCESS;
+}
 
-  const uint32_t N = (uint32_t)sc->count;
-  const uint32_t T = (uint32_t)v.tiles_x * (uint32_t)v.tiles_y;
-  const uint64_t P = (uint64_t)cam->width * cam->height;
+// ---- frame construction ---------------------------------------------------
+
+struct FrameShape {
+  uint32_t nviews;
+  uint64_t tiles;   // frame tiles (sum over views)
+  uint64_t pixels;  // frame pixels (sum over views)
+};
+
+// Per-view parameters on the host and their device copy (through pinned
+// staging, ordered on the stream).
+static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, uint32_t n,
+                                const LumenExRenderConfig* cfg, FrameShape& fs, hipStream_t st) {
+  c->host_views.resize(n);
+  uint64_t tb = 0, pb = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    ViewParams& v = c->host_views[k];
+    fill_view(v, &cams[k], cfg);
+    v.tile_base = (uint32_t)tb;
+    v.view = k;
+    v.pix_base = pb;
+    tb += (uint64_t)v.tiles_x * v.tiles_y;
+    pb += (uint64_t)v.width * v.height;
+  }
+  if (tb >= 0x7FFFFFFFull) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "too many tiles in the frame");
+  fs.nviews = n;
+  fs.tiles = tb;
+  fs.pixels = pb;
+  const size_t bytes = sizeof(ViewParams) * n;
+  if (c->pinned_views_cap < bytes) {
+    if (c->pinned_views) {
+      LUMEN_HIP(hipStreamSynchronize(st));
+      LUMEN_HIP(hipHostFree(c->pinned_views));
+    }
+    c->pinned_views = nullptr;
+    LUMEN_HIP(hipHostMalloc(&c->pinned_views, bytes, hipHostMallocDefault));
+    c->pinned_views_cap = bytes;
+  }
+  memcpy(c->pinned_views, c->host_views.data(), bytes);
+  LUMEN_TRY(ensure(c->views, bytes));
+  LUMEN_TRY(ensure(c->spans, sizeof(ViewSpan) * n));
+  LUMEN_HIP(hipMemcpyAsync(c->views.ptr, c->pinned_views, bytes, hipMemcpyHostToDevice, st));
+  return LUMEN_SUCCESS;
+}
+
+static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t threads,
+                                    hipStream_t st, LumenExContext* c, uint32_t nviews,
+                                    float* rgb, float* depth, float* t_final, int32_t* n_eval) {
+  const int2* ranges = c->ranges.as<int2>();
+  const uint32_t* v0 = c->vals[0].as<uint32_t>();
+  const uint32_t* v1 = c->vals[1].as<uint32_t>();
+  const SortPlan* plan = c->plan.as<SortPlan>();
+  const float4* rec = c->rec.as<float4>();
+  const ViewParams* views = c->views.as<ViewParams>();
+  const ViewSpan* spans = c->spans.as<ViewSpan>();
+  const dim3 grid((uint32_t)tiles), block(threads);
+  switch (batch) {
+    case 64:
+      hipLaunchKernelGGL(k_composite<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         (int)nviews, spans, rgb, depth, t_final, n_eval);
+      break;
+    case 128:
+      hipLaunchKernelGGL(k_composite<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         (int)nviews, spans, rgb, depth, t_final, n_eval);
+      break;
+    default:
+      hipLaunchKernelGGL(k_composite<256>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+                         (int)nviews, spans, rgb, depth, t_final, n_eval);
+      break;
+  }
+  LUMEN_LAUNCHED();
+  return LUMEN_SUCCESS;
+}
+
+// Buffers that depend on N, the views and the pixel count.
+static LumenStatus ensure_frame_buffers(LumenExContext* c, uint64_t N, const FrameShape& fs,
+                                        bool save_pixels) {
+  const uint64_t nv = (uint64_t)fs.nviews * N;
+  LUMEN_TRY(ensure(c->packed, (nv ? nv : 1) * 8));
+  LUMEN_TRY(ensure(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
+  LUMEN_TRY(ensure(c->totals, 16));
+  LUMEN_TRY(ensure(c->bounds, 16));
+  LUMEN_TRY(ensure(c->plan, sizeof(SortPlan)));
+  LUMEN_TRY(ensure(c->ranges, fs.tiles * 8));
+  if (save_pixels) {
+    LUMEN_TRY(ensure(c->t_final, fs.pixels * 4));
+    LUMEN_TRY(ensure(c->n_eval, fs.pixels * 4));
+  }
+  return LUMEN_SUCCESS;
+}
+
+static LumenStatus ensure_capacity(LumenExContext* c, uint64_t v_cap, uint64_t m_cap) {
+  LUMEN_TRY(ensure(c->rec, (v_cap ? v_cap : 1) * 48));
+  LUMEN_TRY(ensure(c->vis2orig, (v_cap ? v_cap : 1) * 4));
+  LUMEN_TRY(ensure(c->radius, (v_cap ? v_cap : 1) * 4));
+  for (int b = 0; b < 2; ++b) {
+    LUMEN_TRY(ensure(c->keys[b], (m_cap ? m_cap : 1) * 8));
+    LUMEN_TRY(ensure(c->vals[b], (m_cap ? m_cap : 1) * 4));
+  }
+  return LUMEN_SUCCESS;
+}
+
+// Stage 1 of a frame: per-(view, Gaussian) counts and their scan.
+static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_init_frame, dim3(1), dim3(1), 0, st, c->bounds.as<uint32_t>(),
+                     c->plan.as<SortPlan>());
+  LUMEN_LAUNCHED();
+  if (sd.n == 0) return LUMEN_SUCCESS;
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
+  hipLaunchKernelGGL(k_count, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
+                     c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
+                     c->bounds.as<uint32_t>());
+  LUMEN_LAUNCHED();
+  const uint64_t nv = (uint64_t)fs.nviews * sd.n;
+  LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
+                                     c->part.as<uint64_t>(), c->totals.as<uint64_t>(), st));
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, true, st));
+  return LUMEN_SUCCESS;
+}
+
+// Stage 2 of a frame: plan, emission, sort, ranges and compositing, with the
+// given capacities (exact after a host read-back, reserved in a graph).
+static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
+                                const LumenExRenderConfig* cfg, uint64_t v_cap, uint64_t m_cap,
+                                int sort_slots, bool keep, float* out_rgb, float* out_depth,
+                                hipStream_t st) {
+  SortPlan* plan = c->plan.as<SortPlan>();
+  if (sd.n > 0) {
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, c->packed.as<uint64_t>(),
+                       c->totals.as<uint64_t>(), c->bounds.as<uint32_t>(), sd.n, (int)fs.nviews,
+                       m_cap, v_cap, plan, c->spans.as<ViewSpan>());
+    LUMEN_LAUNCHED();
+  } else {
+    LUMEN_TRY(fill<uint32_t>(c->spans.as<uint32_t>(), 0u, 4ull * fs.nviews, st));
+  }
+  LUMEN_TRY(fill<int32_t>(c->ranges.as<int32_t>(), -1, 2ull * fs.tiles, st));
+  if (sd.n > 0 && m_cap > 0) {
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, false, st));
+    hipLaunchKernelGGL(k_emit, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
+                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(), m_cap,
+                       v_cap, c->rec.as<float4>(), c->vis2orig.as<int32_t>(),
+                       c->radius.as<int32_t>(), c->keys[0].as<uint64_t>(),
+                       c->vals[0].as<uint32_t>(), plan);
+    LUMEN_LAUNCHED();
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, true, st));
+    if (keep) {
+      LUMEN_TRY(ensure(c->keys_keep, m_cap * 8));
+      LUMEN_TRY(ensure(c->vals_keep, m_cap * 4));
+      LUMEN_HIP(hipMemcpyAsync(c->keys_keep.ptr, c->keys[0].ptr, m_cap * 8,
+                               hipMemcpyDeviceToDevice, st));
+      LUMEN_HIP(hipMemcpyAsync(c->vals_keep.ptr, c->vals[0].ptr, m_cap * 4,
+                               hipMemcpyDeviceToDevice, st));
+    }
+    uint64_t* kb[2] = {c->keys[0].as<uint64_t>(), c->keys[1].as<uint64_t>()};
+    uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
+    LUMEN_TRY(radix_sort_planned(kb, vb, plan, m_cap, sort_slots,
+                                 SortScratch{&c->hist, &c->hist_part}, st));
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
+    hipLaunchKernelGGL(k_ranges, dim3(div_up(m_cap, 256)), dim3(256), 0, st, kb[0], kb[1], plan,
+                       c->ranges.as<int2>(), (uint32_t)fs.tiles);
+    LUMEN_LAUNCHED();
+    LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, true, st));
+  }
+  const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+  float* tf = save ? c->t_final.as<float>() : nullptr;
+  int32_t* ne = save ? c->n_eval.as<int32_t>() : nullptr;
+  const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, false, st));
+  LUMEN_TRY(composite_launch(batch, fs.tiles, cfg->tile_width * cfg->tile_height, st, c,
+                             fs.nviews, out_rgb, out_depth, tf, ne));
+  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, true, st));
+  return LUMEN_SUCCESS;
+}
+
+// Sort passes that can ever be needed for this frame shape: depth keys are
+// at most trunc(far * 1000) (culling keeps depth <= far), frame tile ids at
+// most tiles - 1.
+static int max_sort_slots(const LumenExContext* c, const FrameShape& fs) {
+  float far_max = 0.0f;
+  for (const ViewParams& v : c->host_views) far_max = v.far_plane > far_max ? v.far_plane : far_max;
+  const uint32_t dmax = depth_key(far_max);
+  int32_t dg[8];
+  int n = varying_digits(0, dmax, 0, 4, dg);
+  n += varying_digits(0, fs.tiles ? fs.tiles - 1 : 0, 4, 4, dg + n);
+  return n;
+}
+
+// The plain frame: count, read back the totals, size, render.
+static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
+                                const LumenExCamera* cams, uint32_t nviews,
+                                const LumenExRenderConfig* cfg, float* out_rgb, float* out_depth,
+                                int* passes_out, hipStream_t st) {
+  FrameShape fs;
+  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+  LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
+  const SceneDev sd = scene_dev(sc);
   for (bool& r : c->stage_ran) r = false;
-  c->kept = false;
-  c->pixels = false;
-
-  SceneDev sd{sc->pos_x, sc->pos_y, sc->pos_z, sc->scale_x, sc->scale_y, sc->scale_z, sc->rot_w,
-              sc->rot_x, sc->rot_y, sc->rot_z, sc->opacity, sc->sh, N};
-
+  LUMEN_TRY(enqueue_count(c, sd, fs, st));
   uint64_t V = 0, M = 0;
   uint32_t hb[4] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
-  if (N > 0) {
-    LUMEN_TRY(ensure(c->packed, (uint64_t)N * 8));
-    LUMEN_TRY(ensure(c->part, (uint64_t)div_up(N, kScanChunk) * 8));
-    LUMEN_TRY(ensure(c->totals, 16));
-    LUMEN_TRY(ensure(c->bounds, 16));
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(1), 0, st, c->bounds.as<uint32_t>());
-    LUMEN_LAUNCHED();
-    hipLaunchKernelGGL(k_count, dim3(div_up(N, 256)), dim3(256), 0, st, sd, v,
-                       c->packed.as<uint64_t>(), c->bounds.as<uint32_t>());
-    LUMEN_LAUNCHED();
-    LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), N,
-                                       c->part.as<uint64_t>(), c->totals.as<uint64_t>(), st));
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, true, st));
+  if (sd.n > 0) {
     uint8_t* h = static_cast<uint8_t*>(c->pinned);
     LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 8, hipMemcpyDeviceToHost, st));
     LUMEN_HIP(hipMemcpyAsync(h + 8, c->bounds.ptr, 16, hipMemcpyDeviceToHost, st));
@@ -386,92 +660,130 @@ static LumenStatus render_impl(LumenExContext* c, const LumenExScene* sc, const 
     V = tot & 0xFFFFFFFFull;
     M = tot >> 32;
     if (M >= 0x7FFFFFFFull)
-      return fail(LUMEN_EX_ERROR_CAPACITY, "lumen_ex_render: %llu tile instances exceed 2^31-1",
+      return fail(LUMEN_EX_ERROR_CAPACITY, "%llu tile instances exceed 2^31-1",
                   (unsigned long long)M);
   }
-
-  LUMEN_TRY(ensure(c->ranges, (uint64_t)T * 8));
-  int2* ranges = c->ranges.as<int2>();
-  LUMEN_TRY(fill<int32_t>(c->ranges.as<int32_t>(), -1, 2ull * T, st));
-  int nd = 0;
-  int digits[8];
-  int final_buf = 0;
-  if (M > 0) {
-    LUMEN_TRY(ensure(c->rec, V * 48));
-    LUMEN_TRY(ensure(c->vis2orig, V * 4));
-    LUMEN_TRY(ensure(c->radius, V * 4));
-    for (int b = 0; b < 2; ++b) {
-      LUMEN_TRY(ensure(c->keys[b], M * 8));
-      LUMEN_TRY(ensure(c->vals[b], M * 4));
-    }
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, false, st));
-    hipLaunchKernelGGL(k_emit, dim3(div_up(N, 256)), dim3(256), 0, st, sd, v,
-                       c->packed.as<uint64_t>(), c->rec.as<float4>(), c->vis2orig.as<int32_t>(),
-                       c->radius.as<int32_t>(), c->keys[0].as<uint64_t>(),
-                       c->vals[0].as<uint32_t>());
-    LUMEN_LAUNCHED();
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, true, st));
-    if (cfg->flags & LUMEN_EX_KEEP_STAGES) {
-      LUMEN_TRY(ensure(c->keys_keep, M * 8));
-      LUMEN_TRY(ensure(c->vals_keep, M * 4));
-      LUMEN_HIP(hipMemcpyAsync(c->keys_keep.ptr, c->keys[0].ptr, M * 8, hipMemcpyDeviceToDevice, st));
-      LUMEN_HIP(hipMemcpyAsync(c->vals_keep.ptr, c->vals[0].ptr, M * 4, hipMemcpyDeviceToDevice, st));
-      c->kept = true;
-    }
-    nd = varying_digits(hb[0], hb[1], 0, 4, digits);
-    nd += varying_digits(hb[2], hb[3], 4, 4, digits + nd);
-    uint64_t* kb[2] = {c->keys[0].as<uint64_t>(), c->keys[1].as<uint64_t>()};
-    uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
-    LUMEN_TRY(radix_sort_digits(kb, vb, (uint32_t)M, digits, nd, SortScratch{&c->hist, &c->hist_part},
-                                &final_buf, st));
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
-    hipLaunchKernelGGL(k_ranges, dim3(div_up(M, 256)), dim3(256), 0, st, kb[final_buf], (uint32_t)M,
-                       ranges);
-    LUMEN_LAUNCHED();
-    LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, true, st));
+  LUMEN_TRY(ensure_capacity(c, V, M));
+  int32_t dg[8];
+  int np = 0;
+  if (V > 0) {
+    np = varying_digits(hb[0], hb[1], 0, 4, dg);
+    np += varying_digits(hb[2], hb[3], 4, 4, dg + np);
   }
-
-  float* tf = nullptr;
-  int32_t* ne = nullptr;
-  if (cfg->flags & LUMEN_EX_SAVE_PIXELS) {
-    LUMEN_TRY(ensure(c->t_final, P * 4));
-    LUMEN_TRY(ensure(c->n_eval, P * 4));
-    tf = c->t_final.as<float>();
-    ne = c->n_eval.as<int32_t>();
-    c->pixels = true;
-  }
-  // rasterize_scene returns its zero-filled RenderOutput before the
-  // background fill when there is no tile instance (pipeline.mojo:63-65,
-  // 162-163), so an empty frame is black even with white_background.
-  if (M == 0) v.white_background = 0;
-  const uint32_t* vals_sorted = M > 0 ? c->vals[final_buf].as<uint32_t>() : nullptr;
-  const float4* rec = M > 0 ? c->rec.as<float4>() : nullptr;
-  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, false, st));
-  LUMEN_TRY(composite_launch(batch, T, threads, st, ranges, vals_sorted, rec, v, out_rgb, out_depth,
-                             tf, ne));
-  LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, true, st));
-
+  const bool keep = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0 && M > 0;
+  LUMEN_TRY(enqueue_rest(c, sd, fs, cfg, V, M, np, keep, out_rgb, out_depth, st));
   c->V = V;
   c->M = M;
-  c->T = T;
-  c->P = P;
-  c->N = N;
-  c->width = cam->width;
-  c->height = cam->height;
-  c->tile_w = tw;
-  c->tile_h = th;
-  c->final_buf = final_buf;
+  c->T = fs.tiles;
+  c->P = fs.pixels;
+  c->N = sc->count;
+  c->nviews = nviews;
+  c->tile_w = cfg->tile_width;
+  c->tile_h = cfg->tile_height;
+  c->v_cap = V;
+  c->m_cap = M;
+  c->sizes_known = true;
+  c->kept = keep;
+  c->pixels = save;
   c->has_grad = false;
-  if (stats) {
-    stats->visible_splats = V;
-    stats->total_splat_instances = M;
-    stats->tiles_x = (uint64_t)v.tiles_x;
-    stats->tiles_y = (uint64_t)v.tiles_y;
-    stats->sort_passes = (uint32_t)nd;
-    stats->reserved = 0;
+  if (passes_out) *passes_out = np;
+  return LUMEN_SUCCESS;
+}
+
+static void graph_release(GraphCache& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+  g.valid = false;
+  g.key.clear();
+}
+
+static std::vector<uint8_t> graph_key(const LumenExScene* sc, const LumenExCamera* cams,
+                                      uint32_t n, const LumenExRenderConfig* cfg, float* rgb,
+                                      float* depth, hipStream_t st) {
+  std::vector<uint8_t> k;
+  auto put = [&k](const void* p, size_t b) {
+    const uint8_t* q = static_cast<const uint8_t*>(p);
+    k.insert(k.end(), q, q + b);
+  };
+  put(sc, sizeof(*sc));
+  put(&n, sizeof(n));
+  put(cams, sizeof(LumenExCamera) * n);
+  put(cfg, sizeof(*cfg));
+  put(&rgb, sizeof(rgb));
+  put(&depth, sizeof(depth));
+  put(&st, sizeof(st));
+  return k;
+}
+
+// Graph frame: on a new key, render once plainly (which also sizes the
+// buffers), reserve capacity, capture the whole frame; replay afterwards.
+static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
+                                const LumenExCamera* cams, uint32_t nviews,
+                                const LumenExRenderConfig* cfg, float* out_rgb, float* out_depth,
+                                hipStream_t st) {
+  if (st == nullptr)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "LUMEN_EX_GRAPH needs a non-default stream");
+  std::vector<uint8_t> key = graph_key(sc, cams, nviews, cfg, out_rgb, out_depth, st);
+  if (c->graph.valid && key == c->graph.key) {
+    for (bool& r : c->stage_ran) r = false;
+    LUMEN_HIP(hipGraphLaunch(c->graph.exec, st));
+    c->sizes_known = false;
+    c->kept = false;
+    c->pixels = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+    c->has_grad = false;
+    return LUMEN_SUCCESS;
   }
+  graph_release(c->graph);
+  LumenExRenderConfig pc = *cfg;
+  pc.flags &= ~LUMEN_EX_KEEP_STAGES;
+  LUMEN_TRY(render_plain(c, sc, cams, nviews, &pc, out_rgb, out_depth, nullptr, st));
+  const uint64_t m_cap = c->M + c->M / 4 + 65536;
+  const uint64_t v_cap = (uint64_t)nviews * sc->count;
+  LUMEN_TRY(ensure_capacity(c, v_cap, m_cap));
+  FrameShape fs;
+  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  // sort scratch for the capacity, allocated before capture
+  {
+    const uint32_t nb = div_up(m_cap, kSortTile);
+    const uint64_t hlen = (uint64_t)nb * 256u;
+    LUMEN_TRY(ensure(c->hist, hlen * sizeof(uint32_t)));
+    LUMEN_TRY(ensure(c->hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
+  }
+  LUMEN_HIP(hipStreamSynchronize(st));
+  const int slots = max_sort_slots(c, fs);
+  const SceneDev sd = scene_dev(sc);
+  const bool prof = c->profile;
+  c->profile = false;
+  LUMEN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  LumenStatus s1 = enqueue_count(c, sd, fs, st);
+  LumenStatus s2 = s1 == LUMEN_SUCCESS
+                       ? enqueue_rest(c, sd, fs, &pc, v_cap, m_cap, slots, false, out_rgb,
+                                      out_depth, st)
+                       : s1;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  c->profile = prof;
+  if (s2 != LUMEN_SUCCESS) {
+    if (g) (void)hipGraphDestroy(g);
+    return s2;
+  }
+  if (ec != hipSuccess) return fail(LUMEN_EX_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(LUMEN_EX_ERROR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  }
+  c->graph.graph = g;
+  c->graph.exec = ex;
+  c->graph.key = key;
+  c->graph.valid = true;
+  c->graph.stream = st;
+  c->graph.m_cap = m_cap;
+  c->v_cap = v_cap;
+  c->m_cap = m_cap;
   return LUMEN_SUCCESS;
 }
 
@@ -506,24 +818,92 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
+  graph_release(c->graph);
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->bounds, &c->rec, &c->vis2orig,
-                   &c->radius, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
-                   &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part, &c->ranges,
-                   &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis};
+  DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->bounds, &c->plan, &c->views, &c->spans,
+                   &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
+                   &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
+                   &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis};
   for (DevBuf* b : all) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->pinned_views) (void)hipHostFree(c->pinned_views);
   delete c;
 }
 
-LumenStatus lumen_ex_render(LumenExContext* ctx, const LumenExScene* scene,
-                            const LumenExCamera* camera, const LumenExRenderConfig* config,
-                            float* out_rgb, float* out_depth, LumenExRenderStats* out_stats,
-                            void* stream) {
+LumenStatus lumen_ex_render(LumenExContext* c, const LumenExScene* sc, const LumenExCamera* cam,
+                            const LumenExRenderConfig* cfg, float* out_rgb, float* out_depth,
+                            LumenExRenderStats* stats, void* stream) {
   clear_error();
-  return render_impl(ctx, scene, camera, config, out_rgb, out_depth, out_stats,
-                     (hipStream_t)stream);
+  if (!c || !sc || !cam || !cfg || !out_rgb)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render: null argument");
+  LUMEN_TRY(check_config(cfg));
+  LUMEN_TRY(check_scene(sc));
+  LUMEN_TRY(check_cameras(cam, 1));
+  LUMEN_HIP(hipSetDevice(c->device));
+  if (c->graph.valid) graph_release(c->graph);  // its buffers may be resized below
+  int passes = 0;
+  LUMEN_TRY(render_plain(c, sc, cam, 1, cfg, out_rgb, out_depth, &passes, (hipStream_t)stream));
+  if (stats) {
+    stats->visible_splats = c->V;
+    stats->total_splat_instances = c->M;
+    stats->tiles_x = (uint64_t)c->host_views[0].tiles_x;
+    stats->tiles_y = (uint64_t)c->host_views[0].tiles_y;
+    stats->sort_passes = (uint32_t)passes;
+    stats->reserved = 0;
+  }
+  return LUMEN_SUCCESS;
+}
+
+LumenStatus lumen_ex_render_batch(LumenExContext* c, const LumenExScene* sc,
+                                  const LumenExCamera* cams, uint32_t n_views,
+                                  const LumenExRenderConfig* cfg, float* out_rgb,
+                                  float* out_depth, LumenExViewStats* out_stats, void* stream) {
+  clear_error();
+  if (!c || !sc || !cams || !cfg || !out_rgb)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_render_batch: null argument");
+  LUMEN_TRY(check_config(cfg));
+  LUMEN_TRY(check_scene(sc));
+  LUMEN_TRY(check_cameras(cams, n_views));
+  LUMEN_HIP(hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  if (cfg->flags & LUMEN_EX_GRAPH) {
+    LUMEN_TRY(render_graph(c, sc, cams, n_views, cfg, out_rgb, out_depth, st));
+  } else {
+    if (c->graph.valid) graph_release(c->graph);
+    LUMEN_TRY(render_plain(c, sc, cams, n_views, cfg, out_rgb, out_depth, nullptr, st));
+  }
+  if (out_stats) return lumen_ex_frame_status(c, out_stats, n_views, stream);
+  return LUMEN_SUCCESS;
+}
+
+LumenStatus lumen_ex_frame_status(LumenExContext* c, LumenExViewStats* out_stats, uint32_t n,
+                                  void* stream) {
+  clear_error();
+  if (!c) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_frame_status: null context");
+  hipStream_t st = (hipStream_t)stream;
+  LUMEN_HIP(hipSetDevice(c->device));
+  if (!c->plan.ptr) return fail(LUMEN_ERROR_UNINITIALIZED, "no frame rendered yet");
+  SortPlan p;
+  LUMEN_HIP(hipMemcpyAsync(c->pinned, c->plan.ptr, sizeof(SortPlan), hipMemcpyDeviceToHost, st));
+  LUMEN_HIP(hipStreamSynchronize(st));
+  memcpy(&p, c->pinned, sizeof(SortPlan));
+  if (out_stats && n > 0) {
+    if (n != c->nviews)
+      return fail(LUMEN_ERROR_INVALID_ARGUMENT, "frame has %u views, asked for %u", c->nviews, n);
+    LUMEN_HIP(hipMemcpy(out_stats, c->spans.ptr, sizeof(ViewSpan) * n, hipMemcpyDeviceToHost));
+  }
+  if (!c->sizes_known) {
+    c->V = p.v;
+    c->M = p.m;
+    c->sizes_known = !p.overflow;
+  }
+  if (p.overflow)
+    return fail(LUMEN_EX_ERROR_CAPACITY,
+                "the frame exceeded its reserved key capacity (%llu); render it again without "
+                "LUMEN_EX_GRAPH or with new cameras to re-plan",
+                (unsigned long long)c->m_cap);
+  return LUMEN_SUCCESS;
 }
 
 LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* values, uint64_t n,
@@ -536,7 +916,9 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   if (n <= 1) return LUMEN_SUCCESS;
   hipStream_t st = (hipStream_t)stream;
   LUMEN_HIP(hipSetDevice(c->device));
+  if (c->graph.valid) graph_release(c->graph);
   LUMEN_TRY(ensure(c->minmax, 16));
+  LUMEN_TRY(ensure(c->plan, sizeof(SortPlan)));
   unsigned long long* mm = c->minmax.as<unsigned long long>();
   LUMEN_TRY(fill<unsigned long long>(mm, ~0ull, 1, st));
   LUMEN_TRY(fill<unsigned long long>(mm + 1, 0ull, 1, st));
@@ -547,19 +929,30 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   LUMEN_HIP(hipMemcpyAsync(c->pinned, mm, 16, hipMemcpyDeviceToHost, st));
   LUMEN_HIP(hipStreamSynchronize(st));
   memcpy(hmm, c->pinned, 16);
-  int digits[8];
+  int32_t digits[8];
   const int nd = varying_digits(hmm[0], hmm[1], 0, 8, digits);
   LUMEN_TRY(ensure(c->keys[1], n * 8));
   LUMEN_TRY(ensure(c->vals[1], n * 4));
+  LUMEN_TRY(ensure(c->minmax, 16 + 32));
+  int32_t* ddig = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(c->minmax.ptr) + 16);
+  memcpy(static_cast<uint8_t*>(c->pinned) + 64, digits, sizeof(digits));
+  LUMEN_HIP(hipMemcpyAsync(ddig, static_cast<uint8_t*>(c->pinned) + 64, sizeof(digits),
+                           hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_set_plan, dim3(1), dim3(1), 0, st, c->plan.as<SortPlan>(), nd, ddig,
+                     (uint32_t)n);
+  LUMEN_LAUNCHED();
   uint64_t* kb[2] = {keys, c->keys[1].as<uint64_t>()};
   uint32_t* vb[2] = {values, c->vals[1].as<uint32_t>()};
-  int fb = 0;
-  LUMEN_TRY(radix_sort_digits(kb, vb, (uint32_t)n, digits, nd, SortScratch{&c->hist, &c->hist_part},
-                              &fb, st));
-  if (fb == 1) {
+  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), n, nd,
+                               SortScratch{&c->hist, &c->hist_part}, st));
+  if (nd & 1) {
     LUMEN_HIP(hipMemcpyAsync(keys, kb[1], n * 8, hipMemcpyDeviceToDevice, st));
     LUMEN_HIP(hipMemcpyAsync(values, vb[1], n * 4, hipMemcpyDeviceToDevice, st));
   }
+  LUMEN_HIP(hipStreamSynchronize(st));  // the staging above is reused by the next call
+  c->sizes_known = false;
+  c->M = 0;
+  c->V = 0;
   return LUMEN_SUCCESS;
 }
 
@@ -573,8 +966,8 @@ LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint3
   hipStream_t st = (hipStream_t)stream;
   LUMEN_TRY(fill<int32_t>(ranges, -1, 2ull * tiles, st));
   if (n > 0) {
-    hipLaunchKernelGGL(k_ranges, dim3(div_up(n, 256)), dim3(256), 0, st, sorted_keys, (uint32_t)n,
-                       reinterpret_cast<int2*>(ranges));
+    hipLaunchKernelGGL(k_ranges_plain, dim3(div_up(n, 256)), dim3(256), 0, st, sorted_keys,
+                       (uint32_t)n, reinterpret_cast<int2*>(ranges), tiles);
     LUMEN_LAUNCHED();
   }
   return LUMEN_SUCCESS;
@@ -583,6 +976,10 @@ LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint3
 LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t bytes) {
   clear_error();
   if (!c || !dst) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_copy_stage: null argument");
+  if (!c->sizes_known)
+    return fail(LUMEN_ERROR_UNINITIALIZED,
+                "frame sizes unknown (graph replay): call lumen_ex_frame_status first");
+  const int fb = 0;
   const void* src = nullptr;
   size_t want = 0;
   switch (stage) {
@@ -595,8 +992,22 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t 
     case LUMEN_EX_STAGE_VALS_UNSORTED:
       if (!c->kept && c->M) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_KEEP_STAGES)");
       src = c->vals_keep.ptr; want = c->M * 4; break;
-    case LUMEN_EX_STAGE_KEYS: src = c->keys[c->final_buf].ptr; want = c->M * 8; break;
-    case LUMEN_EX_STAGE_VALS: src = c->vals[c->final_buf].ptr; want = c->M * 4; break;
+    case LUMEN_EX_STAGE_KEYS:
+    case LUMEN_EX_STAGE_VALS: {
+      // the sorted pairs sit in buffer (passes & 1)
+      SortPlan p;
+      if (c->M) {
+        LUMEN_HIP(hipSetDevice(c->device));
+        LUMEN_HIP(hipDeviceSynchronize());
+        LUMEN_HIP(hipMemcpy(&p, c->plan.ptr, sizeof(p), hipMemcpyDeviceToHost));
+      } else {
+        p.passes = fb;
+      }
+      const int b = p.passes & 1;
+      if (stage == LUMEN_EX_STAGE_KEYS) { src = c->keys[b].ptr; want = c->M * 8; }
+      else { src = c->vals[b].ptr; want = c->M * 4; }
+      break;
+    }
     case LUMEN_EX_STAGE_RANGES: src = c->ranges.ptr; want = c->T * 8; break;
     case LUMEN_EX_STAGE_T_FINAL:
       if (!c->pixels) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_SAVE_PIXELS)");

@@ -2,21 +2,25 @@
 // 8-bit digits, the device form of radix_sort_64bit
 // (src/raster/radix_sort.mojo:39-87).
 //
-// One pass per digit that is not constant across all keys (a stable pass over
-// a constant digit is the identity, so skipping it leaves the result
-// unchanged).  Each pass:
+// Only digits that can differ between keys are sorted: a stable pass over a
+// digit that is equal in all keys is the identity.  The digit plan, the
+// number of keys and the buffer holding the result live in device memory
+// (SortPlan), so a frame's whole sort can be captured in a hipGraph with no
+// host round trip; grids are sized for a capacity and surplus blocks exit.
+// Each pass:
 //   k_sort_hist     per-block 256-bin histogram of the digit (LDS atomics on
 //                   per-wave sub-histograms), stored digit-major so that one
-//                   exclusive scan yields every block's start offset per digit
+//                   exclusive scan gives every block's start offset per digit
 //   exclusive_scan  over the 256 x blocks table
-//   k_sort_scatter  stable rank inside the block: elements go round by round
-//                   (256 consecutive elements per round), a wave ranks its 64
-//                   lanes with 8 ballots (lanes with equal digits = peers,
-//                   rank = peers below this lane), waves are ordered through
-//                   LDS counters, so equal digits keep their input order.
+//   k_sort_scatter  stable rank inside the block: 256 consecutive elements
+//                   per round, a wave ranks its 64 lanes with 8 ballots
+//                   (lanes with equal digits are peers; rank = peers below
+//                   this lane), waves are ordered through LDS counters, so
+//                   equal digits keep their input order.
 #pragma once
 
 #include "common.hpp"
+#include "plan.hpp"
 #include "scan.hpp"
 
 namespace lumen {
@@ -25,12 +29,24 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint64_t* __restrict__ keys,
-                                                           uint32_t m, int shift,
-                                                           uint32_t* __restrict__ hist,
-                                                           uint32_t nb) {
+__device__ __forceinline__ uint32_t sort_blocks(uint32_t m) {
+  return (m + kSortTile - 1) / kSortTile;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(uint64_t* k0, uint64_t* k1,
+                                                           const SortPlan* __restrict__ plan,
+                                                           int pass, uint32_t* __restrict__ hist,
+                                                           uint32_t nb_cap) {
   __shared__ uint32_t h[4][256];
+  if (pass >= plan->passes) return;
+  const uint32_t m = plan->m;
   const int t = threadIdx.x;
+  if (blockIdx.x >= sort_blocks(m)) {  // surplus block of a capacity-sized grid
+    hist[(uint32_t)t * nb_cap + blockIdx.x] = 0u;
+    return;
+  }
+  const uint64_t* __restrict__ keys = (pass & 1) ? k1 : k0;
+  const int shift = 8 * plan->digit[pass];
   const int w = t >> 6;
 #pragma unroll
   for (int i = 0; i < 4; ++i) h[i][t] = 0;
@@ -42,19 +58,26 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint64_t* __re
     if (idx < m) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & 0xFFu], 1u);
   }
   __syncthreads();
-  hist[(uint32_t)t * nb + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+  hist[(uint32_t)t * nb_cap + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
-    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t m, int shift, const uint32_t* __restrict__ offsets,
-    uint32_t nb) {
+    uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
+    int pass, const uint32_t* __restrict__ offsets, uint32_t nb_cap) {
   __shared__ uint32_t run[256];
   __shared__ uint32_t wc[4][256];
+  if (pass >= plan->passes) return;
+  const uint32_t m = plan->m;
+  if (blockIdx.x >= sort_blocks(m)) return;
+  const uint64_t* __restrict__ kin = (pass & 1) ? k1 : k0;
+  const uint32_t* __restrict__ vin = (pass & 1) ? v1 : v0;
+  uint64_t* __restrict__ kout = (pass & 1) ? k0 : k1;
+  uint32_t* __restrict__ vout = (pass & 1) ? v0 : v1;
+  const int shift = 8 * plan->digit[pass];
   const int t = threadIdx.x;
   const int w = t >> 6;
   const uint64_t lt = lanemask_lt();
-  run[t] = offsets[(uint32_t)t * nb + blockIdx.x];
+  run[t] = offsets[(uint32_t)t * nb_cap + blockIdx.x];
   const uint32_t base = blockIdx.x * kSortTile;
   for (int j = 0; j < kSortItems; ++j) {
     const uint32_t round = base + j * kSortThreads;
@@ -101,44 +124,29 @@ struct SortScratch {
   DevBuf* hist_part;
 };
 
-// Sorts (k[src], v[src]) over the given digit list, ping-ponging between the
-// two buffers; *final_buf receives the index of the buffer holding the result.
-inline LumenStatus radix_sort_digits(uint64_t* k[2], uint32_t* v[2], uint32_t m, const int* digits,
-                                     int ndigits, SortScratch sc, int* final_buf, hipStream_t st) {
-  int src = 0;
-  if (m > 1 && ndigits > 0) {
-    const uint32_t nb = div_up(m, kSortTile);
-    const uint64_t hlen = (uint64_t)nb * 256u;
-    LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
-    LUMEN_TRY(ensure(*sc.hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
-    uint32_t* hist = sc.hist->as<uint32_t>();
-    uint32_t* part = sc.hist_part->as<uint32_t>();
-    for (int p = 0; p < ndigits; ++p) {
-      const int shift = 8 * digits[p];
-      hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(kSortThreads), 0, st, k[src], m, shift, hist,
-                         nb);
-      LUMEN_LAUNCHED();
-      LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st));
-      hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(kSortThreads), 0, st, k[src], v[src],
-                         k[src ^ 1], v[src ^ 1], m, shift, hist, nb);
-      LUMEN_LAUNCHED();
-      src ^= 1;
-    }
+// Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
+// memory; the result ends in buffer (plan->passes & 1).  m_cap bounds the
+// number of keys the plan may name.
+inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
+                                      uint64_t m_cap, int slots, SortScratch sc,
+                                      hipStream_t st) {
+  if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
+  const uint32_t nb = div_up(m_cap, kSortTile);
+  const uint64_t hlen = (uint64_t)nb * 256u;
+  LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(*sc.hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
+  uint32_t* hist = sc.hist->as<uint32_t>();
+  uint32_t* part = sc.hist_part->as<uint32_t>();
+  for (int p = 0; p < slots; ++p) {
+    hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1], plan, p,
+                       hist, nb);
+    LUMEN_LAUNCHED();
+    LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st));
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1], v[0],
+                       v[1], plan, p, hist, nb);
+    LUMEN_LAUNCHED();
   }
-  *final_buf = src;
   return LUMEN_SUCCESS;
-}
-
-// Digits that can differ between keys lying in [lo, hi]: those at or below the
-// highest bit in which lo and hi differ (all keys in the interval share the
-// bits above it).  Appends digit indices (offset by digit_base) to out.
-inline int varying_digits(uint64_t lo, uint64_t hi, int digit_base, int max_digits, int* out) {
-  uint64_t x = lo ^ hi;
-  int n = 0;
-  if (x == 0) return 0;
-  int top = 63 - __builtin_clzll(x);
-  for (int d = 0; d <= top / 8 && d < max_digits; ++d) out[n++] = digit_base + d;
-  return n;
 }
 
 }  // namespace lumen

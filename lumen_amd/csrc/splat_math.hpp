@@ -104,6 +104,12 @@ struct ViewParams {
   float rot[9];
   int32_t sh_coeffs;  // 1, 4, 9 or 16 (src/math/spherical_harmonics.mojo:144-150)
   int32_t white_background, compute_depth;
+  // Position of this view inside a multi-view frame: its tiles are global
+  // tiles [tile_base, tile_base + tiles_x * tiles_y) and its pixels start at
+  // pix_base in the batch's output planes.
+  uint32_t tile_base;
+  uint32_t view;
+  uint64_t pix_base;
 };
 
 LUMEN_HD void make_view(ViewParams& v, const float eye[3], const float rot_wxyz[4]) {

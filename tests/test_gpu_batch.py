@@ -1,0 +1,159 @@
+"""Multi-view frames (lumen_ex_render_batch), hipGraph replay, batched
+backward and the autotuner.
+
+Every view of a batched frame must equal a single-view render of the same
+camera bit for bit (images, depth, per-view visible/instance counts), which
+the single-view parity tests in turn pin to the oracle.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from lumen_amd.camera import CameraConfig, look_at, orbit
+from lumen_amd.raster import RenderConfig
+from lumen_amd.scene import random_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    from lumen_amd.batch import MultiViewRenderer
+    return MultiViewRenderer(0)
+
+
+def mixed_cameras():
+    cams = []
+    for k, (w, h, f) in enumerate([(320, 240, 300.0), (640, 360, 500.0), (256, 256, 260.0),
+                                   (480, 272, 420.0)]):
+        ang = 2 * math.pi * k / 4
+        cfg = CameraConfig(w, h, f, f, w / 2, h / 2, 0.01, 1000.0)
+        cams.append(look_at(cfg, (4.5 * math.cos(ang), 0.8, 4.5 * math.sin(ang))))
+    return cams
+
+
+def test_batch_views_equal_single_view(renderer, rasterizer, oracle):
+    import torch
+    scene = random_scene(60_000, 1.0, 91)
+    dev = scene.to_device()
+    cams = mixed_cameras()
+    cfg = RenderConfig()
+    say("[batch] 4 mixed-resolution views")
+    out = renderer.render(dev, cams, cfg, want_stats=True)
+    renderer.synchronize()
+    for k, cam in enumerate(cams):
+        single = rasterizer.render(dev, cam, cfg)
+        torch.cuda.synchronize()
+        assert out.stats[k].visible_splats == single.visible_splats
+        assert out.stats[k].total_splat_instances == single.total_splat_instances
+        assert torch.equal(out.views_rgb[k], single.rgb), k
+        assert torch.equal(out.views_depth[k], single.depth), k
+    ref = oracle.rasterize(scene, cams[1], cfg)
+    np.testing.assert_array_equal(out.views_rgb[1].cpu().numpy(), ref["rgb"])
+
+
+def test_graph_replays_match_eager(renderer):
+    """Replays of the captured frame, with eager work and synchronizations in
+    between, each equal the eager (plain) frame."""
+    import torch
+    scene = random_scene(80_000, 1.0, 92)
+    dev = scene.to_device()
+    cams = orbit(CameraConfig(400, 300, 380.0, 380.0, 200.0, 150.0), 6, 5.0)
+    cfg = RenderConfig()
+    eager = renderer.render(dev, cams, cfg, graph=False)
+    renderer.synchronize()
+    ref_rgb, ref_depth = eager.rgb.clone(), eager.depth.clone()
+    for rep in range(4):
+        say(f"[batch] graph replay {rep}")
+        out = renderer.render(dev, cams, cfg, graph=True)
+        st = renderer.status(len(cams))
+        assert sum(s.visible_splats for s in st) > 0
+        junk = torch.randn(1000, 1000, device="cuda") @ torch.randn(1000, 1000, device="cuda")
+        torch.cuda.synchronize()
+        assert torch.equal(out.rgb, ref_rgb), rep
+        assert torch.equal(out.depth, ref_depth), rep
+        del junk
+
+
+def test_graph_capacity_overflow_is_reported(renderer):
+    """A replay whose scene grew past the reserved key capacity reports
+    LUMEN_EX_ERROR_CAPACITY and writes background, without faulting."""
+    import torch
+    from lumen_amd._lib import LumenError
+    scene = random_scene(20_000, 1.0, 93)
+    dev = scene.to_device()
+    cams = orbit(CameraConfig(320, 240, 300.0, 300.0, 160.0, 120.0), 2, 5.0)
+    cfg = RenderConfig()
+    renderer.render(dev, cams, cfg, graph=True)      # plans and captures
+    renderer.render(dev, cams, cfg, graph=True)      # replay
+    renderer.status(2)
+    dev.planes[3:6] *= 8.0                           # far larger splats, same pointers
+    out = renderer.render(dev, cams, cfg, graph=True)
+    with pytest.raises(LumenError) as ei:
+        renderer.status(2)
+    assert ei.value.status == -101
+    renderer.synchronize()
+    assert float(out.rgb.abs().max()) == 0.0
+
+
+def test_batch_backward_is_sum_of_views(renderer, rasterizer):
+    import torch
+    from lumen_amd.batch import rasterize_views
+    from lumen_amd.training import GaussianGradients, rasterize
+    scene = random_scene(30_000, 1.0, 94)
+    cams = mixed_cameras()[:3]
+    cfg = RenderConfig()
+    planes = torch.from_numpy(scene.planes).cuda().requires_grad_(True)
+    sh = torch.from_numpy(scene.sh).cuda().requires_grad_(True)
+    vs = GaussianGradients(scene.count)
+    rgb, depth = rasterize_views(planes, sh, cams, cfg, renderer, viewspace=vs)
+    w_rgb = torch.linspace(-1, 1, rgb.numel(), device="cuda")
+    w_d = torch.linspace(0.5, -0.5, depth.numel(), device="cuda")
+    ((rgb * w_rgb).sum() + (depth * w_d).sum()).backward()
+    torch.cuda.synchronize()
+    gp_b, gs_b = planes.grad.clone(), sh.grad.clone()
+    planes.grad = None
+    sh.grad = None
+    vs1 = GaussianGradients(scene.count)
+    base = 0
+    total = 0
+    for cam in cams:
+        P = cam.config.width * cam.config.height
+        r, d = rasterize(planes, sh, cam, cfg, rasterizer=rasterizer, viewspace=vs1)
+        total = total + (r.reshape(-1) * w_rgb[3 * base: 3 * (base + P)]).sum() + \
+            (d.reshape(-1) * w_d[base: base + P]).sum()
+        base += P
+    total.backward()
+    torch.cuda.synchronize()
+    for a, b in ((gp_b, planes.grad), (gs_b, sh.grad)):
+        err = float((a - b).norm() / b.norm())
+        say(f"[batch] backward rel err {err:.2e}")
+        assert err < 1e-5
+    assert torch.equal(vs.observation_count, vs1.observation_count)
+    assert torch.allclose(vs.viewspace_grad_2d, vs1.viewspace_grad_2d, rtol=1e-5, atol=1e-6)
+
+
+def test_autotune_configs_agree(renderer):
+    import torch
+    from lumen_amd.batch import autotune
+    scene = random_scene(50_000, 1.0, 95)
+    dev = scene.to_device()
+    cams = orbit(CameraConfig(320, 192, 300.0, 300.0, 160.0, 96.0), 3, 5.0)
+    res = autotune(renderer, dev, cams, RenderConfig(), graph=True, reps=2)
+    say(f"[batch] autotune best {res.config.tile_width}x{res.config.tile_height}/"
+        f"{res.config.batch_depth} table {res.table}")
+    assert len(res.table) == 9
+    imgs = []
+    for cfg in (RenderConfig(), RenderConfig(tile_width=8, tile_height=8, batch_depth=64),
+                RenderConfig(tile_width=32, tile_height=8, batch_depth=128)):
+        out = renderer.render(dev, cams, cfg)
+        renderer.synchronize()
+        imgs.append(out.rgb.clone())
+    assert torch.equal(imgs[0], imgs[1]) and torch.equal(imgs[0], imgs[2])
