@@ -10,11 +10,14 @@
 //                     forward computed them (same expressions, same exp), so
 //                     the skip / clamp / termination decisions are the
 //                     forward's; transmittance before each blend is recovered
-//                     from the saved final value by division.  Per-splat
-//                     gradients (mean2d, inverse covariance, opacity, rgb,
-//                     depth) are summed over the wave with shuffles, over the
-//                     block in LDS, and flushed with one atomic per splat,
-//                     component and tile.
+//                     from the saved final value (times 1/(1 - alpha), the
+//                     hardware reciprocal).  A wave skips the splats past all
+//                     of its pixels' last evaluated one and those whose 3-sigma
+//                     box misses its pixels.  Per-splat gradients (mean2d,
+//                     inverse covariance, opacity, rgb, depth) are summed over
+//                     the wave with DPP into a per-wave LDS slot, over the
+//                     waves at the end of each batch, and flushed with one
+//                     atomic per splat, component and tile.
 //   k_preprocess_bwd  one thread per visible Gaussian: adjoints of the SH
 //                     colour (and view direction), the projection to screen,
 //                     the 2D inverse, the EWA projection, and the world
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     const float* __restrict__ g_depth, float* __restrict__ grad_vis) {
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint32_t s_vi[BATCH];
-  __shared__ float s_acc[BATCH * kGradVis];
+  __shared__ float s_part[4][BATCH * kGradVis];  // per-wave sums, one slot per splat
   __shared__ int s_last;
   if (plan->overflow) return;
   const uint32_t gt = blockIdx.x;
@@ -69,16 +72,19 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const ViewParams& v = views[vw];
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int nthreads = v.tile_w * v.tile_h;
+  const int nwaves = nthreads >> 6;
   const int tile = (int)(gt - v.tile_base);
   const int2 rg = ranges[gt];
   if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
   const bool white = v.white_background && spans[vw].instances > 0;
   const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
   const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wv = t >> 6;
   const int px = tx * v.tile_w + t % v.tile_w;
   const int py = ty * v.tile_h + t / v.tile_w;
   const bool inside = px < v.width && py < v.height;
-  for (int e = t; e < BATCH * kGradVis; e += nthreads) s_acc[e] = 0.0f;
+  for (int e = t; e < 4 * BATCH * kGradVis; e += nthreads) (&s_part[0][0])[e] = 0.0f;
   if (t == 0) s_last = rg.x - 1;
   __syncthreads();
   int last = rg.x - 1;
@@ -92,14 +98,21 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     gC2 = g_rgb[3 * pix + 2];
     gD = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
   }
-  atomicMax(&s_last, last);
+  // splats this wave's pixels evaluated: positions [0, wave_cnt) of the list
+  const uint32_t wave_cnt = wave_max_u32_all((uint32_t)(last - rg.x + 1));
+  if (lane == 0) atomicMax(&s_last, rg.x + (int)wave_cnt - 1);
   __syncthreads();
   const int block_last = s_last;
   const float Tf = T;
   const float gsum = gC0 + gC1 + gC2;
   const float pxc = (float)px + 0.5f;
   const float pyc = (float)py + 0.5f;
-  const int lane = t & 63;
+  // pixel-centre extent of this wave (64 / tile_w full rows of the tile)
+  const int wave0 = t & ~63;
+  const float wx0 = (float)(tx * v.tile_w) + 0.5f;
+  const float wx1 = (float)(tx * v.tile_w + v.tile_w) - 0.5f;
+  const float wy0 = (float)(ty * v.tile_h + wave0 / v.tile_w) + 0.5f;
+  const float wy1 = (float)(ty * v.tile_h + (wave0 + 64) / v.tile_w) - 0.5f;
   float SC0 = 0.0f, SC1 = 0.0f, SC2 = 0.0f, SD = 0.0f;
   for (int end = block_last + 1; end > rg.x; end -= BATCH) {
     const int start = max(rg.x, end - BATCH);
@@ -113,12 +126,17 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     }
     __syncthreads();
     for (int j = n - 1; j >= 0; --j) {
+      if ((uint32_t)(start + j - rg.x) >= wave_cnt) continue;  // past every lane's last splat
+      const float4 a = s0[j];
+      const float4 c = s2[j];
+      // Whole wave outside the splat's box: the forward computed power > 4.5
+      // (no blend) at each of these pixels, so the splat has no gradient here.
+      if (a.x + c.z < wx0 || a.x - c.z > wx1 || a.y + c.w < wy0 || a.y - c.w > wy1) continue;
       float gv[kGradVis];
 #pragma unroll
       for (int k = 0; k < kGradVis; ++k) gv[k] = 0.0f;
       bool contrib = false;
       if (start + j <= last) {
-        const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pxc - a.x;
         const float dy = pyc - a.y;
@@ -138,17 +156,17 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           }
           if (!(alpha < kMinAlpha)) {
             contrib = true;
-            const float4 c = s2[j];
             const float om = 1.0f - alpha;
-            const float Tj = T / om;
+            const float rom = __builtin_amdgcn_rcpf(om);
+            const float Tj = T * rom;
             const float w = alpha * Tj;
             gv[6] = gC0 * w;
             gv[7] = gC1 * w;
             gv[8] = gC2 * w;
             gv[9] = gD * w;
-            float dal = gC0 * (b.z * Tj - SC0 / om) + gC1 * (b.w * Tj - SC1 / om) +
-                        gC2 * (c.x * Tj - SC2 / om) + gD * (c.y * Tj - SD / om);
-            if (white) dal -= gsum * Tf / om;
+            float dal = gC0 * (b.z * Tj - SC0 * rom) + gC1 * (b.w * Tj - SC1 * rom) +
+                        gC2 * (c.x * Tj - SC2 * rom) + gD * (c.y * Tj - SD * rom);
+            if (white) dal -= gsum * Tf * rom;
             SC0 += b.z * w;
             SC1 += b.w * w;
             SC2 += c.x * w;
@@ -169,20 +187,23 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           }
         }
       }
-      if (__ballot(contrib)) {  // wave-uniform
+      if (__ballot(contrib)) {  // wave-uniform; the whole wave is active here
 #pragma unroll
-        for (int k = 0; k < kGradVis; ++k) gv[k] = wave_sum(gv[k]);
-        if (lane == 0) {
+        for (int k = 0; k < kGradVis; ++k) gv[k] = wave_sum_lane63(gv[k]);
+        if (lane == 63) {
 #pragma unroll
-          for (int k = 0; k < kGradVis; ++k) atomicAdd(&s_acc[j * kGradVis + k], gv[k]);
+          for (int k = 0; k < kGradVis; ++k) s_part[wv][j * kGradVis + k] = gv[k];
         }
       }
     }
     __syncthreads();
     for (int e = t; e < n * kGradVis; e += nthreads) {
-      const float x = s_acc[e];
+      float x = 0.0f;
+      for (int w = 0; w < nwaves; ++w) {
+        x += s_part[w][e];
+        s_part[w][e] = 0.0f;
+      }
       if (x != 0.0f) atomicAdd(&grad_vis[(size_t)s_vi[e / kGradVis] * kGradVis + e % kGradVis], x);
-      s_acc[e] = 0.0f;
     }
     __syncthreads();
   }
@@ -255,15 +276,23 @@ __device__ __forceinline__ void acc(float* p, float x, bool atomic) {
 
 __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const ViewParams* __restrict__ views,
+                                                        const ViewSpan* __restrict__ spans,
+                                                        int nviews,
                                                         const SortPlan* __restrict__ plan,
-                                                        const float4* __restrict__ rec,
                                                         const int32_t* __restrict__ vis2orig,
                                                         const float* __restrict__ grad_vis,
                                                         GradPlanes out, int atomic_adds) {
   const uint32_t k = blockIdx.x * 256u + threadIdx.x;
   if (k >= plan->v || plan->overflow) return;
   const bool atomic = atomic_adds != 0;
-  const ViewParams& v = views[__float_as_int(rec[3u * k + 2u].z)];
+  // owning view: the last one whose visible run starts at or before k
+  int lo = 0, hi = nviews - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (spans[mid].vis_base <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  const ViewParams& v = views[lo];
   const uint32_t i = (uint32_t)vis2orig[k];
   const float* g = grad_vis + (size_t)k * kGradVis;
   const float gmx = g[0], gmy = g[1], ga = g[2], gb = g[3], gc = g[4];
@@ -280,25 +309,64 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
   normalize3(ux, uy, uz);
   float Y[16];
   sh_basis(ux, uy, uz, Y);
-  const float* sh = s.sh + (size_t)i * 48u;
+  const int ncoef = v.sh_coeffs;
+  const int nfloat = 3 * ncoef;
+  const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+  float c[48];
+#pragma unroll
+  for (int qd = 0; qd < 12; ++qd) {
+    const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+    c[4 * qd + 0] = x.x;
+    c[4 * qd + 1] = x.y;
+    c[4 * qd + 2] = x.z;
+    c[4 * qd + 3] = x.w;
+  }
+  // the forward's colour sums, in its order, for the clamp decision
   float raw[3] = {0.5f, 0.5f, 0.5f};
-  for (int kk = 0; kk < v.sh_coeffs; ++kk)
-    for (int ch = 0; ch < 3; ++ch) raw[ch] += sh[3 * kk + ch] * Y[kk];
+#pragma unroll
+  for (int kk = 0; kk < 16; ++kk)
+    if (kk < ncoef)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) raw[ch] += c[3 * kk + ch] * Y[kk];
   float gcol[3];
+#pragma unroll
   for (int ch = 0; ch < 3; ++ch) gcol[ch] = (raw[ch] < 0.0f || raw[ch] > 1.0f) ? 0.0f : g[6 + ch];
   float dY[16];
-  float* gsh = out.sh + (size_t)i * 48u;
+#pragma unroll
   for (int kk = 0; kk < 16; ++kk) {
     dY[kk] = 0.0f;
-    if (kk >= v.sh_coeffs) continue;
-    for (int ch = 0; ch < 3; ++ch) {
-      acc(&gsh[3 * kk + ch], gcol[ch] * Y[kk], atomic);
-      dY[kk] += gcol[ch] * sh[3 * kk + ch];
+    if (kk < ncoef)
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) dY[kk] += gcol[ch] * c[3 * kk + ch];
+  }
+  // dL/dsh[k][ch] = gcol[ch] * Y[k], written as float4s (elements past the
+  // used coefficients are left as they are)
+  float4* gsh4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
+#pragma unroll
+  for (int qd = 0; qd < 12; ++qd) {
+    if (4 * qd >= nfloat) continue;
+    float add[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = 4 * qd + m;
+      add[m] = gcol[e % 3] * Y[e / 3];
+    }
+    if (atomic) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (4 * qd + m < nfloat) atomicAdd(reinterpret_cast<float*>(gsh4 + qd) + m, add[m]);
+    } else {
+      float4 o = gsh4[qd];
+      if (4 * qd + 0 < nfloat) o.x += add[0];
+      if (4 * qd + 1 < nfloat) o.y += add[1];
+      if (4 * qd + 2 < nfloat) o.z += add[2];
+      if (4 * qd + 3 < nfloat) o.w += add[3];
+      gsh4[qd] = o;
     }
   }
   if (dl > 0.000001f) {
     float gd[3];
-    sh_dir_grad(ux, uy, uz, dY, v.sh_coeffs, gd);
+    sh_dir_grad(ux, uy, uz, dY, ncoef, gd);
     const float dot = gd[0] * ux + gd[1] * uy + gd[2] * uz;
     gpos[0] += (gd[0] - ux * dot) / dl;
     gpos[1] += (gd[1] - uy * dot) / dl;
@@ -456,7 +524,9 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   hipLaunchKernelGGL(k_zero_grad_vis, dim3(zb), dim3(256), 0, st, gv, plan);
   LUMEN_LAUNCHED();
   const uint32_t threads = cfg->tile_width * cfg->tile_height;
-  const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
+  // at most 128 splats per LDS batch: the per-wave gradient slots of 256
+  // would cap occupancy at 3 waves per SIMD
+  const uint32_t batch = cfg->batch_depth == 64u ? 64u : 128u;
   const int2* ranges = c->ranges.as<int2>();
   const uint32_t* v0 = c->vals[0].as<uint32_t>();
   const uint32_t* v1 = c->vals[1].as<uint32_t>();
@@ -473,12 +543,8 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
       hipLaunchKernelGGL(k_composite_bwd<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
                          nv, spans, tf, ne, grad_rgb, grad_depth, gv);
       break;
-    case 128:
-      hipLaunchKernelGGL(k_composite_bwd<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
-      break;
     default:
-      hipLaunchKernelGGL(k_composite_bwd<256>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
+      hipLaunchKernelGGL(k_composite_bwd<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
                          nv, spans, tf, ne, grad_rgb, grad_depth, gv);
       break;
   }
@@ -491,7 +557,8 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   gp.viewspace = grads->viewspace_grad_2d;
   gp.count = grads->observation_count;
   hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(v_cap, 256)), dim3(256), 0, st, scene_dev(sc),
-                     views, plan, rec, c->vis2orig.as<int32_t>(), gv, gp, nv > 1 ? 1 : 0);
+                     views, spans, nv, plan, c->vis2orig.as<int32_t>(), gv, gp,
+                     nv > 1 ? 1 : 0);
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, true, st));
   return LUMEN_SUCCESS;

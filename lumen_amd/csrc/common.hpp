@@ -103,6 +103,46 @@ __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
+// Wave64 reductions on DPP (cross-lane moves inside the VALU, no LDS round
+// trip): row_shr 1, 2, 4, 8 leave each 16-lane row's total in its lane 15,
+// row_bcast:15 and row_bcast:31 fold the four rows into lane 63.  Lanes whose
+// DPP source is out of range (or whose row is masked off) take `identity`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x, uint32_t identity) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Sum over the 64 lanes; the total is valid in lane 63.
+__device__ __forceinline__ float wave_sum_lane63(float x) {
+  x += __uint_as_float(dpp_u32<0x111, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x112, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x114, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x118, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x142, 0xA>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x143, 0xC>(__float_as_uint(x), 0u));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t x) {
+  x = min(x, dpp_u32<0x111, 0xF>(x, 0xFFFFFFFFu));
+  x = min(x, dpp_u32<0x112, 0xF>(x, 0xFFFFFFFFu));
+  x = min(x, dpp_u32<0x114, 0xF>(x, 0xFFFFFFFFu));
+  x = min(x, dpp_u32<0x118, 0xF>(x, 0xFFFFFFFFu));
+  x = min(x, dpp_u32<0x142, 0xA>(x, 0xFFFFFFFFu));
+  x = min(x, dpp_u32<0x143, 0xC>(x, 0xFFFFFFFFu));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32_all(uint32_t x) {
+  x = max(x, dpp_u32<0x111, 0xF>(x, 0u));
+  x = max(x, dpp_u32<0x112, 0xF>(x, 0u));
+  x = max(x, dpp_u32<0x114, 0xF>(x, 0u));
+  x = max(x, dpp_u32<0x118, 0xF>(x, 0u));
+  x = max(x, dpp_u32<0x142, 0xA>(x, 0u));
+  x = max(x, dpp_u32<0x143, 0xC>(x, 0u));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Exclusive scan across the block; every thread must call it.  s_wave must
 // hold THREADS/64 elements.  Ends with a barrier so s_wave may be reused.
 template <typename T, int THREADS>

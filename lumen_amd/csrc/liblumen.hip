@@ -74,11 +74,20 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
   return x;
 }
 
-// Pass 1: visibility and tile count per (view, Gaussian), plus the depth-key
-// and frame-tile-id bounds that decide which sort digits can vary.
+// Geometry of a visible (view, Gaussian) kept by k_count for k_emit (32 B):
+// screen centre, depth, EWA covariance, tile rectangle (16-bit fields).
+struct GeoCache {
+  float4 a;  // sx, sy, depth, c00
+  float4 b;  // c01, c11, rect min (tx | ty << 16), rect max (tx | ty << 16) as bits
+};
+
+// Pass 1: visibility and tile count per (view, Gaussian), the geometry of
+// the visible ones, and per-block bounds of depth keys and frame tile ids
+// (which decide the sort digits; reduced by k_reduce_bounds).
 __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
                                                int nviews, uint64_t* __restrict__ packed,
-                                               uint32_t* __restrict__ bounds) {
+                                               GeoCache* __restrict__ geo,
+                                               uint4* __restrict__ block_bounds) {
   __shared__ uint32_t sb[4][4];
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
@@ -91,6 +100,7 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
       const ViewParams& v = views[vw];
       SplatGeom g;
       uint64_t val = 0;
+      const size_t idx = (size_t)vw * s.n + i;
       if (splat_geometry(v, px, py, pz, s0, s1, s2, q, op, g)) {
         const uint32_t cnt =
             (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
@@ -100,14 +110,20 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
         dmax = max(dmax, dk);
         tmin = min(tmin, v.tile_base + (uint32_t)(g.min_ty * v.tiles_x + g.min_tx));
         tmax = max(tmax, v.tile_base + (uint32_t)(g.max_ty * v.tiles_x + g.max_tx));
+        GeoCache gc;
+        gc.a = make_float4(g.sx, g.sy, g.depth, g.c00);
+        gc.b = make_float4(g.c01, g.c11,
+                           __uint_as_float((uint32_t)g.min_tx | ((uint32_t)g.min_ty << 16)),
+                           __uint_as_float((uint32_t)g.max_tx | ((uint32_t)g.max_ty << 16)));
+        geo[idx] = gc;
       }
-      packed[(size_t)vw * s.n + i] = val;
+      packed[idx] = val;
     }
   }
-  dmin = wave_min_u32(dmin);
-  dmax = wave_max_u32(dmax);
-  tmin = wave_min_u32(tmin);
-  tmax = wave_max_u32(tmax);
+  dmin = wave_min_u32_all(dmin);
+  dmax = wave_max_u32_all(dmax);
+  tmin = wave_min_u32_all(tmin);
+  tmax = wave_max_u32_all(tmax);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sb[w][0] = dmin;
@@ -123,12 +139,46 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
       tmin = min(tmin, sb[k][2]);
       tmax = max(tmax, sb[k][3]);
     }
-    if (dmin != 0xFFFFFFFFu) {
-      atomicMin(&bounds[0], dmin);
-      atomicMax(&bounds[1], dmax);
-      atomicMin(&bounds[2], tmin);
-      atomicMax(&bounds[3], tmax);
+    block_bounds[blockIdx.x] = make_uint4(dmin, dmax, tmin, tmax);
+  }
+}
+
+// One block folds the per-block bounds of k_count into bounds[0..3].
+__global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict__ block_bounds,
+                                                        uint32_t nblocks,
+                                                        uint32_t* __restrict__ bounds) {
+  __shared__ uint32_t sb[16][4];
+  uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) {
+    const uint4 x = block_bounds[b];
+    dmin = min(dmin, x.x);
+    dmax = max(dmax, x.y);
+    tmin = min(tmin, x.z);
+    tmax = max(tmax, x.w);
+  }
+  dmin = wave_min_u32_all(dmin);
+  dmax = wave_max_u32_all(dmax);
+  tmin = wave_min_u32_all(tmin);
+  tmax = wave_max_u32_all(tmax);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sb[w][0] = dmin;
+    sb[w][1] = dmax;
+    sb[w][2] = tmin;
+    sb[w][3] = tmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 16; ++k) {
+      dmin = min(dmin, sb[k][0]);
+      dmax = max(dmax, sb[k][1]);
+      tmin = min(tmin, sb[k][2]);
+      tmax = max(tmax, sb[k][3]);
     }
+    bounds[0] = dmin;
+    bounds[1] = dmax;
+    bounds[2] = tmin;
+    bounds[3] = tmax;
   }
 }
 
@@ -164,85 +214,166 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
   }
 }
 
+// Radius exactly as splat_geometry computes it (solve_eigenvalues_2x2,
+// src/math/cholesky.mojo:287-310), from the cached screen covariance.
+__device__ __forceinline__ int32_t radius_of(float c00, float c01, float c11) {
+  const float mid = (c00 + c11) * 0.5f;
+  const float diff = (c00 - c11) * 0.5f;
+  const float root = sqrtf(diff * diff + c01 * c01);
+  const int64_t rad = trunc_to_i64(ceilf(3.0f * sqrtf(mid + root)));
+  return rad > INT32_MAX ? INT32_MAX : (int32_t)rad;
+}
+
+// Half-widths of a box around the splat centre outside of which the
+// compositor's computed power exceeds 4.5 at every pixel centre (so alpha is
+// 0 there and the splat changes nothing).  In exact arithmetic the power is
+// <= 4.5 only inside the ellipse q <= 9 of the inverse covariance (i00, i01,
+// i11), whose extents are 3 sqrt(i11/det) and 3 sqrt(i00/det); the box is
+// 2% + 0.02 px larger, which covers the rounding of the power for inverse
+// covariances with condition number up to ~1e4.  Worse-conditioned, singular
+// or indefinite inverses get an infinite box (never skipped).
+__device__ __forceinline__ void splat_box(float i00, float i01, float i11, float& hx, float& hy) {
+  const float det = i00 * i11 - i01 * i01;
+  const float tr = i00 + i11;
+  if (!(i00 > 0.0f) || !(i11 > 0.0f) || !(det > 0.0f) || !(tr * tr <= 1.0e4f * det)) {
+    hx = __builtin_inff();
+    hy = __builtin_inff();
+    return;
+  }
+  hx = 3.0f * sqrtf(i11 / det) * 1.02f + 0.02f;
+  hy = 3.0f * sqrtf(i00 / det) * 1.02f + 0.02f;
+}
+
 // Pass 2: records and keys of every visible (view, Gaussian) at its scanned
-// offsets.  The SH block is read once per Gaussian, for the first view that
-// sees it.
+// offsets, from the geometry k_count cached.  Keys are written by the whole
+// wave together: the wave's keys are one contiguous run (the scan is in
+// Gaussian order), each lane takes every 64th key of the run, finds the
+// splat that owns it by binary search over the lanes' start offsets in LDS,
+// and derives the tile from its rectangle -- coalesced stores and no lane
+// waiting on another lane's long rectangle.  Emission order within a splat
+// is ty-major then tx (pipeline.mojo:152-157).
 __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __restrict__ views,
                                               int nviews, const uint64_t* __restrict__ offsets,
-                                              uint64_t m_cap, uint64_t v_cap,
-                                              float4* __restrict__ rec,
+                                              const uint64_t* __restrict__ totals,
+                                              const GeoCache* __restrict__ geo, uint64_t m_cap,
+                                              uint64_t v_cap, float4* __restrict__ rec,
                                               int32_t* __restrict__ vis2orig,
                                               int32_t* __restrict__ radius,
                                               uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals,
                                               SortPlan* __restrict__ plan) {
+  __shared__ uint32_t s_start[4][64], s_rmin[4][64], s_rmax[4][64], s_dk[4][64], s_vi[4][64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= s.n) return;
-  const float px = s.px[i], py = s.py[i], pz = s.pz[i];
-  const float s0 = s.sx[i], s1 = s.sy[i], s2 = s.sz[i];
-  const Quat q{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
-  const float op = s.op[i];
+  const bool valid = i < s.n;
+  const uint64_t nv = (uint64_t)nviews * s.n;
+  float px = 0.0f, py = 0.0f, pz = 0.0f, op = 0.0f;
+  if (valid) {
+    px = s.px[i];
+    py = s.py[i];
+    pz = s.pz[i];
+    op = s.op[i];
+  }
   bool have_sh = false;
   float c[48];
   for (int vw = 0; vw < nviews; ++vw) {
     const ViewParams& v = views[vw];
-    SplatGeom g;
-    if (!splat_geometry(v, px, py, pz, s0, s1, s2, q, op, g)) continue;
-    const uint64_t off = offsets[(size_t)vw * s.n + i];
-    const uint32_t vi = (uint32_t)off;
-    uint32_t ko = (uint32_t)(off >> 32);
-    const uint32_t cnt = (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
-    if ((uint64_t)ko + cnt > m_cap || (uint64_t)vi >= v_cap) {
-      atomicOr(&plan->overflow, 1u);
-      continue;
+    uint32_t vis = 0, cnt = 0, vi = 0, ko = 0;
+    if (valid) {
+      const uint64_t idx = (uint64_t)vw * s.n + i;
+      const uint64_t off0 = offsets[idx];
+      const uint64_t off1 = idx + 1 < nv ? offsets[idx + 1] : *totals;
+      vis = (uint32_t)off1 - (uint32_t)off0;
+      cnt = (uint32_t)(off1 >> 32) - (uint32_t)(off0 >> 32);
+      vi = (uint32_t)off0;
+      ko = (uint32_t)(off0 >> 32);
     }
-    if (!have_sh) {
-      const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
-      const int nfloat = 3 * v.sh_coeffs;
+    // The wave's run of keys is [base, end); lane 0 is valid whenever any lane is.
+    const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)ko);
+    const uint32_t end = wave_max_u32_all(valid ? ko + cnt : 0u);
+    const uint32_t vend = wave_max_u32_all(vis ? vi + 1u : 0u);
+    const bool overflow = (uint64_t)end > m_cap || (uint64_t)vend > v_cap;
+    if (overflow && lane == 0) atomicOr(&plan->overflow, 1u);
+    uint32_t rmin = 0, rmax = 0, dk = 0;
+    if (vis && !overflow) {
+      const GeoCache gc = geo[(uint64_t)vw * s.n + i];
+      const float sx = gc.a.x, sy = gc.a.y, depth = gc.a.z;
+      const float c00 = gc.a.w, c01 = gc.b.x, c11 = gc.b.y;
+      rmin = __float_as_uint(gc.b.z);
+      rmax = __float_as_uint(gc.b.w);
+      if (!have_sh) {
+        const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+        const int nfloat = 3 * v.sh_coeffs;
 #pragma unroll
-      for (int qd = 0; qd < 12; ++qd) {
-        const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
-        c[4 * qd + 0] = x.x;
-        c[4 * qd + 1] = x.y;
-        c[4 * qd + 2] = x.z;
-        c[4 * qd + 3] = x.w;
+        for (int qd = 0; qd < 12; ++qd) {
+          const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+          c[4 * qd + 0] = x.x;
+          c[4 * qd + 1] = x.y;
+          c[4 * qd + 2] = x.z;
+          c[4 * qd + 3] = x.w;
+        }
+        have_sh = true;
       }
-      have_sh = true;
-    }
-    // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
-    // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139
-    float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
-    normalize3(dx, dy, dz);
-    float Y[16];
-    sh_basis(dx, dy, dz, Y);
-    float cr = 0.5f, cg = 0.5f, cb = 0.5f;
+      // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
+      // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139
+      float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
+      normalize3(dx, dy, dz);
+      float Y[16];
+      sh_basis(dx, dy, dz, Y);
+      float cr = 0.5f, cg = 0.5f, cb = 0.5f;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k < v.sh_coeffs) {
-        cr += c[3 * k + 0] * Y[k];
-        cg += c[3 * k + 1] * Y[k];
-        cb += c[3 * k + 2] * Y[k];
+      for (int k = 0; k < 16; ++k) {
+        if (k < v.sh_coeffs) {
+          cr += c[3 * k + 0] * Y[k];
+          cg += c[3 * k + 1] * Y[k];
+          cb += c[3 * k + 2] * Y[k];
+        }
       }
+      if (cr < 0.0f) cr = 0.0f; else if (cr > 1.0f) cr = 1.0f;
+      if (cg < 0.0f) cg = 0.0f; else if (cg > 1.0f) cg = 1.0f;
+      if (cb < 0.0f) cb = 0.0f; else if (cb > 1.0f) cb = 1.0f;
+      float i00, i01, i11, hx, hy;
+      inverse2d(c00, c01, c11, i00, i01, i11);
+      splat_box(i00, i01, i11, hx, hy);
+      rec[3u * vi + 0u] = make_float4(sx, sy, i00, 2.0f * i01);
+      rec[3u * vi + 1u] = make_float4(i11, op, cr, cg);
+      rec[3u * vi + 2u] = make_float4(cb, depth, hx, hy);
+      vis2orig[vi] = (int32_t)i;
+      radius[vi] = radius_of(c00, c01, c11);
+      dk = depth_key(depth);
     }
-    if (cr < 0.0f) cr = 0.0f; else if (cr > 1.0f) cr = 1.0f;
-    if (cg < 0.0f) cg = 0.0f; else if (cg > 1.0f) cg = 1.0f;
-    if (cb < 0.0f) cb = 0.0f; else if (cb > 1.0f) cb = 1.0f;
-    float i00, i01, i11;
-    inverse2d(g.c00, g.c01, g.c11, i00, i01, i11);
-    rec[3u * vi + 0u] = make_float4(g.sx, g.sy, i00, 2.0f * i01);
-    rec[3u * vi + 1u] = make_float4(i11, op, cr, cg);
-    rec[3u * vi + 2u] = make_float4(cb, g.depth, __int_as_float((int)vw), 0.0f);
-    vis2orig[vi] = (int32_t)i;
-    radius[vi] = g.radius;
-    // Emission order: ty-major then tx (pipeline.mojo:152-157).
-    const uint64_t dk = depth_key(g.depth);
-    for (int ty = g.min_ty; ty <= g.max_ty; ++ty) {
-      for (int tx = g.min_tx; tx <= g.max_tx; ++tx) {
-        keys[ko] = ((uint64_t)(v.tile_base + (uint32_t)(ty * v.tiles_x + tx)) << 32) | dk;
-        vals[ko] = vi;
-        ++ko;
+    if (overflow) continue;  // uniform across the wave
+    s_start[w][lane] = valid ? ko - base : 0xFFFFFFFFu;
+    s_rmin[w][lane] = rmin;
+    s_rmax[w][lane] = rmax;
+    s_dk[w][lane] = dk;
+    s_vi[w][lane] = vi;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t total = end - base;
+    for (uint32_t p = lane; p < total; p += 64) {
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_start[w][mid] <= p) lo = mid;
+        else hi = mid - 1;
       }
+      const uint32_t local = p - s_start[w][lo];
+      const uint32_t r0 = s_rmin[w][lo], r1 = s_rmax[w][lo];
+      const uint32_t min_tx = r0 & 0xFFFFu, min_ty = r0 >> 16;
+      const uint32_t wtiles = (r1 & 0xFFFFu) - min_tx + 1u;
+      const uint32_t row = local / wtiles;
+      const uint32_t ty = min_ty + row;
+      const uint32_t tx = min_tx + (local - row * wtiles);
+      keys[base + p] = ((uint64_t)(v.tile_base + ty * (uint32_t)v.tiles_x + tx) << 32) |
+                       (uint64_t)s_dk[w][lo];
+      vals[base + p] = s_vi[w][lo];
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -311,10 +442,17 @@ __global__ __launch_bounds__(256) void k_composite(
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
   float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f, D = 0.0f;
-  int32_t ne = 0;
+  // evaluations = position of the terminating splat + 1, or the list length
+  int32_t ne = rg.y - rg.x;
   if (rg.x != -1 && rg.x < rg.y) {
     const float pxc = (float)px + 0.5f;
     const float pyc = (float)py + 0.5f;
+    // pixel-centre extent of this wave (64 / tile_w full rows of the tile)
+    const int wave0 = t & ~63;
+    const float wx0 = (float)(tx * v.tile_w) + 0.5f;
+    const float wx1 = (float)(tx * v.tile_w + v.tile_w) - 0.5f;
+    const float wy0 = (float)(ty * v.tile_h + wave0 / v.tile_w) + 0.5f;
+    const float wy1 = (float)(ty * v.tile_h + (wave0 + 64) / v.tile_w) - 0.5f;
     bool done = !inside;
     for (int base = rg.x; base < rg.y; base += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
@@ -329,19 +467,22 @@ __global__ __launch_bounds__(256) void k_composite(
       if (!done) {
         for (int j = 0; j < n; ++j) {
           const float4 a = s0[j];
+          const float4 c = s2[j];
+          // Whole wave outside the splat's box: every lane would compute
+          // power > 4.5 (alpha 0, no blend), so the splat is skipped.
+          if (a.x + c.z < wx0 || a.x - c.z > wx1 || a.y + c.w < wy0 || a.y - c.w > wy1)
+            continue;
           const float4 b = s1[j];
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
           const float dy = pyc - a.y;
           float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
-          ++ne;
           if (power < 0.0f) power = 0.0f;
           if (power > 4.5f) continue;
           float alpha = b.y * splat_exp(-power);
           if (alpha > 0.99f) alpha = 0.99f;
           // PixelAccumulator.blend, compositor.mojo:25-53
           if (alpha < kMinAlpha) continue;
-          const float4 c = s2[j];
           const float wgt = alpha * T;
           C0 += b.z * wgt;
           C1 += b.w * wgt;
@@ -350,6 +491,7 @@ __global__ __launch_bounds__(256) void k_composite(
           T *= (1.0f - alpha);
           if (T <= kTransmittanceEps) {
             done = true;
+            ne = base + j - rg.x + 1;
             break;
           }
         }
@@ -412,9 +554,10 @@ static bool plane_ok(const float* p) { return p != nullptr; }
 static LumenStatus check_config(const LumenExRenderConfig* cfg) {
   const uint32_t tw = cfg->tile_width, th = cfg->tile_height;
   const uint32_t threads = tw * th;
-  if (tw == 0 || th == 0 || threads < 64 || threads > 256 || threads % 64 != 0)
+  if (tw == 0 || th == 0 || threads < 64 || threads > 256 || threads % 64 != 0 || 64 % tw != 0)
     return fail(LUMEN_ERROR_INVALID_ARGUMENT,
-                "tile %ux%u unsupported (pixels per tile must be 64, 128, 192 or 256)", tw, th);
+                "tile %ux%u unsupported (64, 128, 192 or 256 pixels per tile, width dividing 64)",
+                tw, th);
   const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
   if (batch != 64 && batch != 128 && batch != 256)
     return fail(LUMEN_ERROR_INVALID_ARGUMENT, "batch_depth %u not in {64,128,256}", batch);
@@ -524,6 +667,8 @@ static LumenStatus ensure_frame_buffers(LumenExContext* c, uint64_t N, const Fra
                                         bool save_pixels) {
   const uint64_t nv = (uint64_t)fs.nviews * N;
   LUMEN_TRY(ensure(c->packed, (nv ? nv : 1) * 8));
+  LUMEN_TRY(ensure(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
+  LUMEN_TRY(ensure(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 16));
   LUMEN_TRY(ensure(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
   LUMEN_TRY(ensure(c->totals, 16));
   LUMEN_TRY(ensure(c->bounds, 16));
@@ -555,9 +700,13 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
   LUMEN_LAUNCHED();
   if (sd.n == 0) return LUMEN_SUCCESS;
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
-  hipLaunchKernelGGL(k_count, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
-                     c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
-                     c->bounds.as<uint32_t>());
+  const uint32_t nblocks = div_up(sd.n, 256);
+  hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(256), 0, st, sd, c->views.as<ViewParams>(),
+                     (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(),
+                     c->block_bounds.as<uint4>());
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, st, c->block_bounds.as<uint4>(),
+                     nblocks, c->bounds.as<uint32_t>());
   LUMEN_LAUNCHED();
   const uint64_t nv = (uint64_t)fs.nviews * sd.n;
   LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
@@ -585,7 +734,8 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
   if (sd.n > 0 && m_cap > 0) {
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, false, st));
     hipLaunchKernelGGL(k_emit, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
-                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(), m_cap,
+                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
+                       c->totals.as<uint64_t>(), c->geo.as<GeoCache>(), m_cap,
                        v_cap, c->rec.as<float4>(), c->vis2orig.as<int32_t>(),
                        c->radius.as<int32_t>(), c->keys[0].as<uint64_t>(),
                        c->vals[0].as<uint32_t>(), plan);
@@ -822,6 +972,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->bounds, &c->plan, &c->views, &c->spans,
+                   &c->geo, &c->block_bounds,
                    &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
                    &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
                    &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis};

@@ -1,0 +1,258 @@
+"""Render server: resident scenes, HBM-budget admission, continuous batching.
+
+Scenes stay resident in HBM.  Requests (scene, camera, forward or
+forward+backward) arrive over time.  The scheduler owns S slots; a slot is a
+lumen_ex context with its own HIP stream, so up to S frames run on the device
+at once.  Whenever a slot is free it admits, from the requests that have
+arrived, a frame:
+  - forward requests of one scene are merged into one multi-view frame
+    (lumen_ex_render_batch; resolutions may differ), up to max_views;
+  - a forward+backward request runs as a frame of its own (its gradient is
+    its own), loss 0.8 L1 + 0.2 (1 - SSIM) against the request's target.
+A frame is admitted only while the sum of the admitted frames' memory
+estimates stays within the HBM budget; requests that do not fit wait.  When
+a frame's completion event fires its memory and slot are released, and every
+request in it records its completion time.  Latency = completion - arrival.
+
+Each view of a batched frame is bit-identical to rendering that camera alone
+(lumen_ex_render_batch guarantees it), so a request's result does not depend
+on what it was batched or overlapped with; tests check this.
+"""
+from __future__ import annotations
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .batch import MultiViewRenderer
+from .camera import PinholeCamera
+from .raster import Rasterizer, RenderConfig
+from .scene import DeviceScene
+
+
+@dataclass
+class Request:
+    rid: int
+    scene: str
+    camera: PinholeCamera
+    backward: bool = False
+    target: Optional[torch.Tensor] = None   # (H, W, 3) for forward+backward
+    arrival: float = 0.0                    # seconds since the server started
+    # filled by the server
+    admitted: float = -1.0
+    done: float = -1.0
+    rgb: Optional[torch.Tensor] = None
+    depth: Optional[torch.Tensor] = None
+    loss: Optional[float] = None
+    grad_planes: Optional[torch.Tensor] = None
+    grad_sh: Optional[torch.Tensor] = None
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+    @property
+    def pixels(self) -> int:
+        return self.camera.config.width * self.camera.config.height
+
+
+@dataclass
+class _Frame:
+    slot: int
+    requests: List[Request]
+    mem: int
+    event: torch.cuda.Event
+    keep: list = field(default_factory=list)
+
+
+def frame_memory_estimate(scene: DeviceScene, cams: Sequence[PinholeCamera],
+                          backward: bool) -> int:
+    """Upper-bound style estimate of the device memory a frame touches
+    beyond the resident scene: per-(view, Gaussian) counts, records, keys
+    (8 B key + 4 B value, double buffered, ~4 tiles per visible splat),
+    outputs and, for a backward, gradient planes of the scene's size."""
+    n = scene.count
+    v = len(cams) * n
+    pix = sum(c.config.width * c.config.height for c in cams)
+    m = 4 * v
+    mem = 8 * v + 56 * v + 24 * m + 24 * pix
+    if backward:
+        mem += scene.nbytes() + 40 * v + 12 * pix
+    return int(mem)
+
+
+class RenderServer:
+    def __init__(self, scenes: Dict[str, DeviceScene], hbm_budget: int = 32 << 30,
+                 slots: int = 3, max_views: int = 16, config: Optional[RenderConfig] = None,
+                 device: int = 0):
+        self.scenes = scenes
+        self.budget = int(hbm_budget)
+        self.config = config or RenderConfig()
+        self.max_views = max_views
+        self.device = device
+        self.slots = [MultiViewRenderer(device) for _ in range(slots)]
+        self.free = list(range(slots))
+        self.inflight: List[_Frame] = []
+        self.mem_in_use = 0
+        self.queue: collections.deque = collections.deque()
+        self.completed: List[Request] = []
+        self.t0 = 0.0
+        self.frames = 0
+
+    # -- scheduling ---------------------------------------------------------
+    def _now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def _pick_frame(self) -> Optional[List[Request]]:
+        """The oldest waiting request and, if it is a forward request, the other
+        waiting forward requests of its scene (arrival order)."""
+        if not self.queue:
+            return None
+        head = self.queue[0]
+        if head.backward:
+            return [head]
+        group = [r for r in self.queue if not r.backward and r.scene == head.scene]
+        return group[: self.max_views]
+
+    def _admit(self) -> bool:
+        if not self.free:
+            return False
+        group = self._pick_frame()
+        if not group:
+            return False
+        scene = self.scenes[group[0].scene]
+        mem = frame_memory_estimate(scene, [r.camera for r in group], group[0].backward)
+        if self.mem_in_use + mem > self.budget and self.inflight:
+            return False  # wait for memory; a lone frame is always admitted
+        for r in group:
+            self.queue.remove(r)
+        slot = self.free.pop()
+        now = self._now()
+        for r in group:
+            r.admitted = now
+        frame = self._launch(slot, scene, group)
+        frame.mem = mem
+        self.mem_in_use += mem
+        self.inflight.append(frame)
+        self.frames += 1
+        return True
+
+    def _launch(self, slot: int, scene: DeviceScene, group: List[Request]) -> _Frame:
+        R = self.slots[slot]
+        cams = [r.camera for r in group]
+        if group[0].backward:
+            r = group[0]
+            from .batch import rasterize_views
+            from .training import combined_loss
+            with torch.cuda.stream(R.stream):
+                planes = scene.planes.detach().requires_grad_(True)
+                sh = scene.sh.detach().requires_grad_(True)
+                rgb, depth = rasterize_views(planes, sh, cams, self.config, R)
+                H, W = r.camera.config.height, r.camera.config.width
+                img = rgb.view(H, W, 3)
+                loss = combined_loss(img, r.target)
+                loss.backward()
+                r.rgb, r.depth = img.detach(), depth.detach().view(H, W)
+                r.grad_planes, r.grad_sh = planes.grad, sh.grad
+                r.loss = loss.detach()
+                ev = torch.cuda.Event()
+                ev.record(R.stream)
+            return _Frame(slot, group, 0, ev)
+        out = R.render(scene, cams, self.config)
+        ev = torch.cuda.Event()
+        ev.record(R.stream)
+        for r, rgb, dep in zip(group, out.views_rgb, out.views_depth):
+            r.rgb, r.depth = rgb, dep
+        # the outputs are the slot's reusable planes: copy them out on the slot's
+        # stream before the slot can be reused
+        with torch.cuda.stream(R.stream):
+            for r in group:
+                r.rgb = r.rgb.clone()
+                r.depth = r.depth.clone()
+            ev.record(R.stream)
+        return _Frame(slot, group, 0, ev)
+
+    def _retire(self) -> int:
+        n = 0
+        for f in list(self.inflight):
+            if f.event.query():
+                now = self._now()
+                for r in f.requests:
+                    r.done = now
+                    if r.loss is not None and isinstance(r.loss, torch.Tensor):
+                        r.loss = float(r.loss)
+                    self.completed.append(r)
+                self.inflight.remove(f)
+                self.mem_in_use -= f.mem
+                self.free.append(f.slot)
+                n += 1
+        return n
+
+    # -- driving --------------------------------------------------------------
+    def run(self, requests: Sequence[Request], realtime: bool = True,
+            timeout: float = 600.0) -> List[Request]:
+        """Serve requests whose `arrival` times are offsets in seconds from now
+        (realtime) or all at once (realtime=False).  Returns them completed."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        self.completed = []
+        self.t0 = time.perf_counter()
+        i = 0
+        while len(self.completed) < len(pending):
+            now = self._now()
+            if now > timeout:
+                raise TimeoutError("render server did not drain its requests")
+            while i < len(pending) and (not realtime or pending[i].arrival <= now):
+                if not realtime:
+                    pending[i].arrival = now
+                self.queue.append(pending[i])
+                i += 1
+            progressed = self._retire() > 0
+            while self._admit():
+                progressed = True
+            if not progressed:
+                time.sleep(0.0002)
+        torch.cuda.synchronize()
+        return self.completed
+
+    def serve_one(self, request: Request) -> Request:
+        """A request alone (no batching, nothing else in flight)."""
+        request.arrival = 0.0
+        return self.run([request], realtime=False)[0]
+
+
+def poisson_requests(scene_names: Sequence[str], resolutions, rate: float, count: int,
+                     backward_fraction: float = 0.2, seed: int = 5,
+                     camera_radius=(4.0, 7.0), targets: Optional[Callable] = None) -> List[Request]:
+    """count requests with exponential inter-arrival times (rate per second),
+    a uniformly random scene and resolution each, fwd+bwd with the given
+    probability, cameras looking at the origin from random directions."""
+    from .camera import CameraConfig, look_at
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += rng.exponential(1.0 / rate)
+        name = scene_names[int(rng.integers(len(scene_names)))]
+        w, h, f = resolutions[int(rng.integers(len(resolutions)))]
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        eye = d * rng.uniform(*camera_radius)
+        cam = look_at(CameraConfig(w, h, f, f, w / 2.0, h / 2.0, 0.01, 1000.0), eye)
+        bwd = bool(rng.uniform() < backward_fraction)
+        req = Request(k, name, cam, bwd, None, t)
+        if bwd and targets is not None:
+            req.target = targets(req)
+        out.append(req)
+    return out
+
+
+def latency_summary(done: Sequence[Request], wall: float) -> dict:
+    lat = np.array([r.latency for r in done]) * 1e3
+    return {"requests": len(done), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "throughput_rps": round(len(done) / wall, 2)}
