@@ -190,11 +190,15 @@ typedef struct {
 
 /* Backward of the last lumen_ex_render on this context, which must have
  * been called with LUMEN_EX_SAVE_PIXELS and the same scene, camera and config.
- * grad_rgb: H*W*3 floats, grad_depth: H*W floats or NULL (device).  The
- * compositor's reverse pass recomputes every alpha exactly as the forward
- * did, recovers transmittance by division from the saved final value, and
- * accumulates per-splat gradients; a second kernel applies the EWA, SH and
- * covariance adjoints per visible Gaussian. */
+ * grad_rgb: H*W*3 floats, grad_depth: H*W floats or NULL (device).  Gradients
+ * are ADDED to the planes of grads.  The compositor's reverse pass recomputes
+ * every alpha exactly as the forward did, recovers transmittance from the
+ * saved final value, and stores each (splat, tile) instance's partial
+ * gradient in its own slot; a second kernel sums a splat's slots in a fixed
+ * order and a third applies the EWA, SH and covariance adjoints per visible
+ * Gaussian.
+ * No atomics are involved for one view, so the result is the same on every
+ * run and does not depend on what else runs on the device. */
 LumenStatus lumen_ex_render_backward(LumenExContext* ctx, const LumenExScene* scene,
                                      const LumenExCamera* camera,
                                      const LumenExRenderConfig* config, const float* grad_rgb,
@@ -203,8 +207,9 @@ LumenStatus lumen_ex_render_backward(LumenExContext* ctx, const LumenExScene* sc
 
 /* Backward of the last lumen_ex_render_batch (same cameras and config, with
  * LUMEN_EX_SAVE_PIXELS); grad_rgb/grad_depth in the batch's output layout.
- * Works after a graph replay too.  Gradients of a Gaussian seen in several
- * views are summed (atomically, so the summation order is not fixed). */
+ * Works after a graph replay too.  Each view's contribution is computed as
+ * for one view; the contributions of a Gaussian seen in several views are
+ * added atomically, so their order (only) is not fixed. */
 LumenStatus lumen_ex_render_batch_backward(LumenExContext* ctx, const LumenExScene* scene,
                                            const LumenExCamera* cameras, uint32_t n_views,
                                            const LumenExRenderConfig* config,

@@ -2,7 +2,7 @@
 //
 // The reference has no backward (src/optim/trainer.mojo:59-65 injects dummy
 // gradients); the output layout is its GaussianGradients
-// (src/optim/gradients.mojo:10-106).  Two kernels:
+// (src/optim/gradients.mojo:10-106).  Three kernels:
 //
 //   k_composite_bwd   one workgroup per tile, one lane per pixel, walking the
 //                     tile's sorted list BACK to front from each pixel's last
@@ -15,9 +15,14 @@
 //                     of its pixels' last evaluated one and those whose 3-sigma
 //                     box misses its pixels.  Per-splat gradients (mean2d,
 //                     inverse covariance, opacity, rgb, depth) are summed over
-//                     the wave with DPP into a per-wave LDS slot, over the
-//                     waves at the end of each batch, and flushed with one
-//                     atomic per splat, component and tile.
+//                     the wave (gfx950 lane swaps + DPP) into a per-wave LDS
+//                     slot and over the waves at the end of each batch, then
+//                     stored (no atomics) in the slot of that (splat, tile) instance in emission
+//                     order: the splat's first key offset plus the tile's
+//                     position in its rectangle, row-major.
+//   k_sum_partial     sums each visible splat's instance slots in a fixed
+//                     order (so the result does not depend on scheduling and
+//                     is the same run to run).
 //   k_preprocess_bwd  one thread per visible Gaussian: adjoints of the SH
 //                     colour (and view direction), the projection to screen,
 //                     the 2D inverse, the EWA projection, and the world
@@ -47,8 +52,8 @@ __device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ vie
   return lo;
 }
 
-__global__ void k_zero_grad_vis(float* __restrict__ g, const SortPlan* __restrict__ plan) {
-  const uint64_t n = (uint64_t)plan->v * kGradVis;
+__global__ void k_zero_partial(float* __restrict__ g, const SortPlan* __restrict__ plan) {
+  const uint64_t n = (uint64_t)plan->m * kGradVis;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
     g[i] = 0.0f;
@@ -59,11 +64,15 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
     const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
-    const ViewSpan* __restrict__ spans, const float* __restrict__ t_final,
-    const int32_t* __restrict__ n_eval, const float* __restrict__ g_rgb,
-    const float* __restrict__ g_depth, float* __restrict__ grad_vis) {
+    const ViewSpan* __restrict__ spans, const uint4* __restrict__ vmeta,
+    const float* __restrict__ t_final, const int32_t* __restrict__ n_eval,
+    const float* __restrict__ g_rgb, const float* __restrict__ g_depth,
+    float* __restrict__ partial) {
+  static_assert(BATCH <= 256, "list entries are bytes");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
-  __shared__ uint32_t s_vi[BATCH];
+  __shared__ uint32_t s_dst[BATCH];  // emission-order slot of each staged instance
+  __shared__ uint8_t s_mask[BATCH];
+  __shared__ uint8_t s_list[4][BATCH];
   __shared__ float s_part[4][BATCH * kGradVis];  // per-wave sums, one slot per splat
   __shared__ int s_last;
   if (plan->overflow) return;
@@ -73,6 +82,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int nthreads = v.tile_w * v.tile_h;
   const int nwaves = nthreads >> 6;
+  const int rows = 64 / v.tile_w;
   const int tile = (int)(gt - v.tile_base);
   const int2 rg = ranges[gt];
   if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
@@ -107,36 +117,47 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const float gsum = gC0 + gC1 + gC2;
   const float pxc = (float)px + 0.5f;
   const float pyc = (float)py + 0.5f;
-  // pixel-centre extent of this wave (64 / tile_w full rows of the tile)
-  const int wave0 = t & ~63;
-  const float wx0 = (float)(tx * v.tile_w) + 0.5f;
-  const float wx1 = (float)(tx * v.tile_w + v.tile_w) - 0.5f;
-  const float wy0 = (float)(ty * v.tile_h + wave0 / v.tile_w) + 0.5f;
-  const float wy1 = (float)(ty * v.tile_h + (wave0 + 64) / v.tile_w) - 0.5f;
   float SC0 = 0.0f, SC1 = 0.0f, SC2 = 0.0f, SD = 0.0f;
   for (int end = block_last + 1; end > rg.x; end -= BATCH) {
     const int start = max(rg.x, end - BATCH);
     const int n = end - start;
     for (int j = t; j < n; j += nthreads) {
       const uint32_t vi = vals[start + j];
-      s_vi[j] = vi;
-      s0[j] = rec[3u * vi + 0u];
+      const uint4 mt = vmeta[vi];  // first key, rect min, rect max (tx | ty << 16)
+      const uint32_t min_tx = mt.y & 0xFFFFu, min_ty = mt.y >> 16;
+      const uint32_t wtiles = (mt.z & 0xFFFFu) - min_tx + 1u;
+      s_dst[j] = mt.x + ((uint32_t)ty - min_ty) * wtiles + ((uint32_t)tx - min_tx);
+      const float4 a = rec[3u * vi + 0u];
+      const float4 c = rec[3u * vi + 2u];
+      s0[j] = a;
       s1[j] = rec[3u * vi + 1u];
-      s2[j] = rec[3u * vi + 2u];
+      s2[j] = c;
+      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
+                                      v.tile_w, rows, nwaves);
     }
     __syncthreads();
-    for (int j = n - 1; j >= 0; --j) {
-      if ((uint32_t)(start + j - rg.x) >= wave_cnt) continue;  // past every lane's last splat
-      const float4 a = s0[j];
-      const float4 c = s2[j];
-      // Whole wave outside the splat's box: the forward computed power > 4.5
-      // (no blend) at each of these pixels, so the splat has no gradient here.
-      if (a.x + c.z < wx0 || a.x - c.z > wx1 || a.y + c.w < wy0 || a.y - c.w > wy1) continue;
+    // this wave's list: splats up to its pixels' last evaluated one whose box
+    // meets its strip (the others have no gradient at its pixels), in order
+    uint32_t cnt = 0;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+      const int j = c0 + lane;
+      const bool hit = j < n && (uint32_t)(start + j - rg.x) < wave_cnt &&
+                       ((s_mask[j] >> wv) & 1u);
+      const uint64_t bal = __ballot(hit);
+      if (hit) s_list[wv][cnt + __popcll(bal & lanemask_lt())] = (uint8_t)j;
+      cnt += __popcll(bal);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int q = (int)cnt - 1; q >= 0; --q) {  // back to front; every lane runs every q
+      const int j = s_list[wv][q];
       float gv[kGradVis];
 #pragma unroll
       for (int k = 0; k < kGradVis; ++k) gv[k] = 0.0f;
       bool contrib = false;
       if (start + j <= last) {
+        const float4 a = s0[j];
         const float4 b = s1[j];
         const float dx = pxc - a.x;
         const float dy = pyc - a.y;
@@ -147,7 +168,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           pclamp = true;
         }
         if (!(power > 4.5f)) {
-          const float G = splat_exp(-power);
+          const float G = splat_exp_neg(-power);
           float alpha = b.y * G;
           bool aclamp = false;
           if (alpha > 0.99f) {
@@ -156,6 +177,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           }
           if (!(alpha < kMinAlpha)) {
             contrib = true;
+            const float4 c = s2[j];
             const float om = 1.0f - alpha;
             const float rom = __builtin_amdgcn_rcpf(om);
             const float Tj = T * rom;
@@ -188,11 +210,14 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         }
       }
       if (__ballot(contrib)) {  // wave-uniform; the whole wave is active here
-#pragma unroll
-        for (int k = 0; k < kGradVis; ++k) gv[k] = wave_sum_lane63(gv[k]);
-        if (lane == 63) {
-#pragma unroll
-          for (int k = 0; k < kGradVis; ++k) s_part[wv][j * kGradVis + k] = gv[k];
+        float sa, sb, sc;
+        wave_sum10(gv, sa, sb, sc);
+        if ((lane & 15) == 15) {
+          const int r = lane >> 4;
+          float* dst = &s_part[wv][j * kGradVis];
+          dst[sum10_slot(r)] = sa;
+          dst[4 + sum10_slot(r)] = sb;
+          if ((r & 1) == 0) dst[8 + (r >> 1)] = sc;
         }
       }
     }
@@ -203,9 +228,77 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         x += s_part[w][e];
         s_part[w][e] = 0.0f;
       }
-      if (x != 0.0f) atomicAdd(&grad_vis[(size_t)s_vi[e / kGradVis] * kGradVis + e % kGradVis], x);
+      if (x != 0.0f) partial[(size_t)s_dst[e / kGradVis] * kGradVis + e % kGradVis] = x;
     }
     __syncthreads();
+  }
+}
+
+// Sum of each visible splat's tile-instance partials (its emission-order
+// slots [first key, next splat's first key)), in a fixed order so the result
+// does not depend on scheduling: a lane sums a splat of up to kSumLane
+// instances sequentially; the wave takes larger splats one at a time, lane l
+// summing instances l, l + 64, ... and the wave folding the 64 lane sums with
+// DPP.  One thread per visible splat.
+constexpr uint32_t kSumLane = 32;
+
+__global__ __launch_bounds__(256) void k_sum_partial(const uint4* __restrict__ vmeta,
+                                                     const float* __restrict__ partial,
+                                                     const SortPlan* __restrict__ plan,
+                                                     float* __restrict__ grad_vis) {
+  if (plan->overflow) return;
+  const uint32_t V = plan->v, M = plan->m;
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t q0 = 0, q1 = 0;
+  if (k < V) {
+    q0 = vmeta[k].x;
+    q1 = k + 1u < V ? vmeta[k + 1u].x : M;
+  }
+  const bool big = q1 - q0 > kSumLane;
+  float g[kGradVis];
+#pragma unroll
+  for (int c = 0; c < kGradVis; ++c) g[c] = 0.0f;
+  if (!big) {
+    for (uint32_t q = q0; q < q1; ++q) {
+      const float2* pp = reinterpret_cast<const float2*>(partial + (size_t)q * kGradVis);
+#pragma unroll
+      for (int c = 0; c < kGradVis / 2; ++c) {
+        const float2 x = pp[c];
+        g[2 * c] += x.x;
+        g[2 * c + 1] += x.y;
+      }
+    }
+  }
+  uint64_t todo = __ballot(big);
+  while (todo) {  // wave-uniform
+    const int src = __builtin_ctzll(todo);
+    todo &= todo - 1ull;
+    const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)q0, src);
+    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)q1, src);
+    float h[kGradVis];
+#pragma unroll
+    for (int c = 0; c < kGradVis; ++c) h[c] = 0.0f;
+    for (uint32_t q = b0 + (uint32_t)lane; q < b1; q += 64u) {
+      const float2* pp = reinterpret_cast<const float2*>(partial + (size_t)q * kGradVis);
+#pragma unroll
+      for (int c = 0; c < kGradVis / 2; ++c) {
+        const float2 x = pp[c];
+        h[2 * c] += x.x;
+        h[2 * c + 1] += x.y;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kGradVis; ++c) {
+      const float tot = wave_sum_lane63(h[c]);
+      const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), 63));
+      if (lane == src) g[c] = b;
+    }
+  }
+  if (k < V) {
+    float2* out = reinterpret_cast<float2*>(grad_vis + (size_t)k * kGradVis);
+#pragma unroll
+    for (int c = 0; c < kGradVis / 2; ++c) out[c] = make_float2(g[2 * c], g[2 * c + 1]);
   }
 }
 
@@ -517,11 +610,14 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   // replay (the kernels read the frame's own counts from the device plan).
   const uint64_t v_cap = c->sizes_known ? c->V : c->v_cap;
   if (sc->count == 0 || v_cap == 0 || (c->sizes_known && c->M == 0)) return LUMEN_SUCCESS;
+  const uint64_t m_cap = c->sizes_known ? c->M : c->m_cap;
+  LUMEN_TRY(ensure(c->partial, (m_cap ? m_cap : 1) * kGradVis * sizeof(float)));
   LUMEN_TRY(ensure(c->grad_vis, v_cap * kGradVis * sizeof(float)));
-  const SortPlan* plan = c->plan.as<SortPlan>();
   float* gv = c->grad_vis.as<float>();
-  const uint32_t zb = div_up(v_cap * kGradVis, 256) < 4096 ? div_up(v_cap * kGradVis, 256) : 4096;
-  hipLaunchKernelGGL(k_zero_grad_vis, dim3(zb), dim3(256), 0, st, gv, plan);
+  const SortPlan* plan = c->plan.as<SortPlan>();
+  float* part = c->partial.as<float>();
+  const uint32_t zb = div_up(m_cap * kGradVis, 1024) < 8192 ? div_up(m_cap * kGradVis, 1024) : 8192;
+  hipLaunchKernelGGL(k_zero_partial, dim3(zb ? zb : 1), dim3(256), 0, st, part, plan);
   LUMEN_LAUNCHED();
   const uint32_t threads = cfg->tile_width * cfg->tile_height;
   // at most 128 splats per LDS batch: the per-wave gradient slots of 256
@@ -533,6 +629,7 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   const float4* rec = c->rec.as<float4>();
   const ViewParams* views = c->views.as<ViewParams>();
   const ViewSpan* spans = c->spans.as<ViewSpan>();
+  const uint4* vmeta = c->vmeta.as<uint4>();
   const float* tf = c->t_final.as<float>();
   const int32_t* ne = c->n_eval.as<int32_t>();
   const int nv = (int)c->nviews;
@@ -541,11 +638,11 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   switch (batch) {
     case 64:
       hipLaunchKernelGGL(k_composite_bwd<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
+                         nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
       break;
     default:
       hipLaunchKernelGGL(k_composite_bwd<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         nv, spans, tf, ne, grad_rgb, grad_depth, gv);
+                         nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
       break;
   }
   LUMEN_LAUNCHED();
@@ -556,6 +653,9 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   gp.sh = grads->sh;
   gp.viewspace = grads->viewspace_grad_2d;
   gp.count = grads->observation_count;
+  hipLaunchKernelGGL(k_sum_partial, dim3(div_up(v_cap, 256)), dim3(256), 0, st, vmeta, part, plan,
+                     gv);
+  LUMEN_LAUNCHED();
   hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(v_cap, 256)), dim3(256), 0, st, scene_dev(sc),
                      views, spans, nv, plan, c->vis2orig.as<int32_t>(), gv, gp,
                      nv > 1 ? 1 : 0);

@@ -123,6 +123,45 @@ __device__ __forceinline__ float wave_sum_lane63(float x) {
   return x;
 }
 
+// Sum over each 16-lane row; the row's total is valid in its lane 15.
+__device__ __forceinline__ float row_sum_lane15(float x) {
+  x += __uint_as_float(dpp_u32<0x111, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x112, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x114, 0xF>(__float_as_uint(x), 0u));
+  x += __uint_as_float(dpp_u32<0x118, 0xF>(__float_as_uint(x), 0u));
+  return x;
+}
+
+// gfx950 v_permlane32_swap: lanes 0-31 get lo[l] + lo[l + 32], lanes 32-63
+// get hi[l - 32] + hi[l] (the upper half of lo trades places with the lower
+// half of hi, then the two registers are added).
+__device__ __forceinline__ float swap32_add(float lo, float hi) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false,
+                                                  false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// gfx950 v_permlane16_swap: with rows of 16 lanes, row 0 gets x.row0 +
+// x.row1, row 1 y.row0 + y.row1, row 2 x.row2 + x.row3, row 3 y.row2 + y.row3.
+__device__ __forceinline__ float swap16_add(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false,
+                                                  false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// Sums of ten per-lane values over the whole wave by transposition: two
+// lane-swap rounds fold the values pairwise (each add now serves two values),
+// then DPP finishes within 16-lane rows.  Every lane must be active.  On
+// return, lane 16 r + 15 holds the sum of value sum10_slot(r) in a, of value
+// 4 + sum10_slot(r) in b, and (r = 0, 2) of value 8 + r / 2 in c.
+__device__ __forceinline__ int sum10_slot(int r) { return (r & 1) * 2 + (r >> 1); }  // 0 2 1 3
+__device__ __forceinline__ void wave_sum10(const float v[10], float& a, float& b, float& c) {
+  a = row_sum_lane15(swap16_add(swap32_add(v[0], v[1]), swap32_add(v[2], v[3])));
+  b = row_sum_lane15(swap16_add(swap32_add(v[4], v[5]), swap32_add(v[6], v[7])));
+  const float r = swap32_add(v[8], v[9]);
+  c = row_sum_lane15(swap16_add(r, r));
+}
+
 __device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t x) {
   x = min(x, dpp_u32<0x111, 0xF>(x, 0xFFFFFFFFu));
   x = min(x, dpp_u32<0x112, 0xF>(x, 0xFFFFFFFFu));

@@ -73,9 +73,11 @@ struct LumenExContext {
   int device = 0;
   lumen::DevBuf packed, part, totals, bounds, plan, views, spans, geo, block_bounds;
   lumen::DevBuf rec, vis2orig, radius;
+  lumen::DevBuf vmeta;    // per visible splat: first key offset, tile rectangle (uint4)
   lumen::DevBuf keys[2], vals[2], keys_keep, vals_keep;
   lumen::DevBuf hist, hist_part;
   lumen::DevBuf ranges, t_final, n_eval, minmax;
+  lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
   void* pinned = nullptr;       // small host staging (totals, bounds, plan)
   void* pinned_views = nullptr; // host staging of the per-view parameters

@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
                                               uint64_t v_cap, float4* __restrict__ rec,
                                               int32_t* __restrict__ vis2orig,
                                               int32_t* __restrict__ radius,
+                                              uint4* __restrict__ vmeta,
                                               uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals,
                                               SortPlan* __restrict__ plan) {
@@ -341,6 +342,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       rec[3u * vi + 2u] = make_float4(cb, depth, hx, hy);
       vis2orig[vi] = (int32_t)i;
       radius[vi] = radius_of(c00, c01, c11);
+      vmeta[vi] = make_uint4(ko, rmin, rmax, 0u);
       dk = depth_key(depth);
     }
     if (overflow) continue;  // uniform across the wave
@@ -417,8 +419,10 @@ __device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views
 
 // One workgroup per tile, one lane per pixel (lane t -> (t % tile_w, t /
 // tile_w), so a wave covers 64/tile_w full rows).  Splat records of the
-// tile's sorted range are staged through LDS BATCH at a time; every lane
-// walks the batch front to back exactly as PixelAccumulator does
+// tile's sorted range are staged through LDS BATCH at a time with the strip
+// mask of each; every wave then compacts the batch to the splats whose box
+// meets its strip (ballot + prefix count, order kept) and each lane walks
+// that list front to back exactly as PixelAccumulator does
 // (src/raster/compositor.mojo:25-89).  A lane stops at T <= 1e-4; the block
 // stops when all its lanes have.
 template <int BATCH>
@@ -428,14 +432,21 @@ __global__ __launch_bounds__(256) void k_composite(
     const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
     const ViewSpan* __restrict__ spans, float* __restrict__ rgb, float* __restrict__ depth,
     float* __restrict__ t_final, int32_t* __restrict__ n_eval) {
+  static_assert(BATCH <= 256, "list entries are bytes");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
+  __shared__ uint8_t s_mask[BATCH];
+  __shared__ uint8_t s_list[4][BATCH];
   const uint32_t gt = blockIdx.x;
   const int vw = view_of_tile(views, nviews, gt);
   const ViewParams& v = views[vw];
   const int nthreads = v.tile_w * v.tile_h;
+  const int nwaves = nthreads >> 6;
+  const int rows = 64 / v.tile_w;
   const int tile = (int)(gt - v.tile_base);
   const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
   const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
   const int px = tx * v.tile_w + t % v.tile_w;
   const int py = ty * v.tile_h + t / v.tile_w;
   const bool inside = px < v.width && py < v.height;
@@ -447,31 +458,37 @@ __global__ __launch_bounds__(256) void k_composite(
   if (rg.x != -1 && rg.x < rg.y) {
     const float pxc = (float)px + 0.5f;
     const float pyc = (float)py + 0.5f;
-    // pixel-centre extent of this wave (64 / tile_w full rows of the tile)
-    const int wave0 = t & ~63;
-    const float wx0 = (float)(tx * v.tile_w) + 0.5f;
-    const float wx1 = (float)(tx * v.tile_w + v.tile_w) - 0.5f;
-    const float wy0 = (float)(ty * v.tile_h + wave0 / v.tile_w) + 0.5f;
-    const float wy1 = (float)(ty * v.tile_h + (wave0 + 64) / v.tile_w) - 0.5f;
     bool done = !inside;
     for (int base = rg.x; base < rg.y; base += BATCH) {
       if (__syncthreads_count(!done) == 0) break;
       const int n = min(BATCH, rg.y - base);
       for (int j = t; j < n; j += nthreads) {
         const uint32_t vi = vals[base + j];
-        s0[j] = rec[3u * vi + 0u];
+        const float4 a = rec[3u * vi + 0u];
+        const float4 c = rec[3u * vi + 2u];
+        s0[j] = a;
         s1[j] = rec[3u * vi + 1u];
-        s2[j] = rec[3u * vi + 2u];
+        s2[j] = c;
+        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
+                                        v.tile_w, rows, nwaves);
       }
       __syncthreads();
+      // this wave's list of the batch's splats, in order
+      uint32_t cnt = 0;
+      for (int c0 = 0; c0 < n; c0 += 64) {
+        const int j = c0 + lane;
+        const bool hit = j < n && ((s_mask[j] >> w) & 1u);
+        const uint64_t bal = __ballot(hit);
+        if (hit) s_list[w][cnt + __popcll(bal & lanemask_lt())] = (uint8_t)j;
+        cnt += __popcll(bal);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (!done) {
-        for (int j = 0; j < n; ++j) {
+        for (uint32_t q = 0; q < cnt; ++q) {
+          const int j = s_list[w][q];
           const float4 a = s0[j];
-          const float4 c = s2[j];
-          // Whole wave outside the splat's box: every lane would compute
-          // power > 4.5 (alpha 0, no blend), so the splat is skipped.
-          if (a.x + c.z < wx0 || a.x - c.z > wx1 || a.y + c.w < wy0 || a.y - c.w > wy1)
-            continue;
           const float4 b = s1[j];
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
@@ -479,10 +496,11 @@ __global__ __launch_bounds__(256) void k_composite(
           float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
           if (power < 0.0f) power = 0.0f;
           if (power > 4.5f) continue;
-          float alpha = b.y * splat_exp(-power);
+          float alpha = b.y * splat_exp_neg(-power);
           if (alpha > 0.99f) alpha = 0.99f;
           // PixelAccumulator.blend, compositor.mojo:25-53
           if (alpha < kMinAlpha) continue;
+          const float4 c = s2[j];
           const float wgt = alpha * T;
           C0 += b.z * wgt;
           C1 += b.w * wgt;
@@ -685,6 +703,7 @@ static LumenStatus ensure_capacity(LumenExContext* c, uint64_t v_cap, uint64_t m
   LUMEN_TRY(ensure(c->rec, (v_cap ? v_cap : 1) * 48));
   LUMEN_TRY(ensure(c->vis2orig, (v_cap ? v_cap : 1) * 4));
   LUMEN_TRY(ensure(c->radius, (v_cap ? v_cap : 1) * 4));
+  LUMEN_TRY(ensure(c->vmeta, (v_cap ? v_cap : 1) * 16));
   for (int b = 0; b < 2; ++b) {
     LUMEN_TRY(ensure(c->keys[b], (m_cap ? m_cap : 1) * 8));
     LUMEN_TRY(ensure(c->vals[b], (m_cap ? m_cap : 1) * 4));
@@ -737,7 +756,7 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                        c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
                        c->totals.as<uint64_t>(), c->geo.as<GeoCache>(), m_cap,
                        v_cap, c->rec.as<float4>(), c->vis2orig.as<int32_t>(),
-                       c->radius.as<int32_t>(), c->keys[0].as<uint64_t>(),
+                       c->radius.as<int32_t>(), c->vmeta.as<uint4>(), c->keys[0].as<uint64_t>(),
                        c->vals[0].as<uint32_t>(), plan);
     LUMEN_LAUNCHED();
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, true, st));
@@ -975,7 +994,8 @@ void lumen_ex_context_destroy(LumenExContext* c) {
                    &c->geo, &c->block_bounds,
                    &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
                    &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
-                   &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis};
+                   &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis,
+                   &c->vmeta, &c->partial};
   for (DevBuf* b : all) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);

@@ -310,4 +310,41 @@ LUMEN_HD float splat_exp(float x) {
   return ldexpf(p, (int)k);
 }
 
+// splat_exp on the compositor's domain x = -power, power in [0, 4.5] (or
+// NaN): the same operations without the range branches, so identical bits
+// there (NaN in gives NaN out).
+LUMEN_HD float splat_exp_neg(float x) {
+  const float k = rintf(x * 1.44269504f);
+  float r = fmaf(k, -0.693145751953125f, x);
+  r = fmaf(k, -1.42860677e-06f, r);
+  float p = 1.98412698e-4f;
+  p = fmaf(p, r, 1.38888889e-3f);
+  p = fmaf(p, r, 8.33333333e-3f);
+  p = fmaf(p, r, 4.16666667e-2f);
+  p = fmaf(p, r, 1.66666667e-1f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
+
+// Bit w of the result: the splat's box (centre +- (hx, hy), outside of which
+// its power exceeds 4.5 at every pixel centre) meets the pixel centres of
+// wave w's strip of the tile (64 / tile_w full rows).  A wave whose bit is
+// clear computes power > 4.5 (alpha 0, no blend) for the splat at each of
+// its pixels, so skipping the splat there changes nothing.
+__device__ __forceinline__ uint32_t strip_mask(float sx, float sy, float hx, float hy, int tile_x0,
+                                               int tile_y0, int tile_w, int rows, int nwaves) {
+  const float x0 = (float)tile_x0 + 0.5f;
+  const float x1 = (float)(tile_x0 + tile_w) - 0.5f;
+  if (sx + hx < x0 || sx - hx > x1) return 0u;
+  uint32_t m = 0u;
+  for (int w = 0; w < nwaves; ++w) {
+    const float y0 = (float)(tile_y0 + w * rows) + 0.5f;
+    const float y1 = (float)(tile_y0 + w * rows + rows) - 0.5f;
+    if (!(sy + hy < y0 || sy - hy > y1)) m |= 1u << w;
+  }
+  return m;
+}
+
 }  // namespace lumen

@@ -177,3 +177,30 @@ def test_fwd_bwd_1m_1080p_descends(rasterizer):
         loss2 = combined_loss(rgb2, target)
     say(f"[bwd] 1M loss {float(loss):.6f} -> {float(loss2):.6f}")
     assert float(loss2) < float(loss)
+
+
+def test_backward_is_deterministic(rasterizer):
+    """One view: every partial gradient has its own slot and is summed in a
+    fixed order, so repeated backward passes agree bit for bit."""
+    import torch
+    from lumen_amd.training import GaussianGradients, rasterize
+    scene = random_scene(100_000, 1.0, 41)
+    cfg_cam = CameraConfig(640, 480, 600.0, 600.0, 320.0, 240.0)
+    cam = look_at(cfg_cam, (0.0, 0.5, 4.5))
+    w = None
+    results = []
+    for rep in range(3):
+        planes = torch.from_numpy(scene.planes).cuda().requires_grad_(True)
+        sh = torch.from_numpy(scene.sh).cuda().requires_grad_(True)
+        vs = GaussianGradients(scene.count)
+        rgb, depth = rasterize(planes, sh, cam, RenderConfig(), rasterizer=rasterizer, viewspace=vs)
+        if w is None:
+            g = torch.Generator(device="cuda").manual_seed(3)
+            w = torch.randn(rgb.shape, generator=g, device="cuda")
+        (rgb * w).sum().backward()
+        torch.cuda.synchronize()
+        results.append((planes.grad.clone(), sh.grad.clone(), vs.viewspace_grad_2d.clone()))
+        say(f"[bwd] determinism rep {rep}")
+    for r in results[1:]:
+        for a, b in zip(results[0], r):
+            assert torch.equal(a, b)

@@ -54,11 +54,10 @@ def test_results_identical_alone_and_in_the_mix(scenes):
         solo = alone_server.serve_one(Request(r.rid, r.scene, r.camera, r.backward, r.target))
         assert torch.equal(solo.rgb, r.rgb), r.rid
         assert torch.equal(solo.depth, r.depth), r.rid
-        if r.backward:
+        if r.backward:  # one view: the backward is deterministic
             assert solo.loss == r.loss
-            for a, b in ((solo.grad_planes, r.grad_planes), (solo.grad_sh, r.grad_sh)):
-                err = float((a - b).norm() / max(float(b.norm()), 1e-30))
-                assert err < 1e-5, (r.rid, err)
+            assert torch.equal(solo.grad_planes, r.grad_planes), r.rid
+            assert torch.equal(solo.grad_sh, r.grad_sh), r.rid
 
 
 def test_budget_admission_serialises_but_completes(scenes):
