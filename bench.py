@@ -1,28 +1,48 @@
 #!/usr/bin/env python3
 """Benchmark of the splat rasterization path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--legs fwd_bwd,batch,serve]
+                    [--no-cpu-baseline] [--copy-calibration]
 
-Workload (default): 1M Gaussians (means N(0, 2 I), log-scales N(ln .01, .5),
+Headline workload: 1M Gaussians (means N(0, 2 I), log-scales N(ln .01, .5),
 random unit quaternions, opacity sigmoid(N(0, 1.5)), FP32 SH degree 3),
-1920x1080, fx = fy = 1400, camera at (0, 0, 5) looking at the origin.
-A step renders one frame (culling, covariance, EWA, SH, key emission, radix
+1920x1080, fx = fy = 1400, camera at (0, 0, 5) looking at the origin.  A
+step renders one frame (culling, covariance, EWA, SH, key emission, radix
 sort, tile ranges, compositing) from a scene resident in HBM into FP32
 RGB + depth buffers in HBM.  Prints one JSON line.
+
+Legs beside the headline (each in its own key of the line):
+  fwd_bwd  the same frame forward + backward through torch.autograd with
+           loss 0.8 L1 + 0.2 (1 - SSIM 8x8) against a render of a copy whose
+           means moved by N(0, 0.01); steps/s.
+  batch    3M Gaussians, 32 1920x1080 cameras on a Catmull-Rom orbit of
+           radius 6: every view in one hipGraph-captured frame (tile shape and
+           LDS batch depth from the autotuner), forward and forward+backward,
+           against the plain per-view path on the same views; views/s.
+  serve    8 resident scenes (10k .. 5M Gaussians), 1000 requests, each a
+           random scene, resolution in {320x240, 1280x720, 1920x1080,
+           3840x2160} and 20% forward+backward: saturation throughput of the
+           continuously batching server against one-at-a-time plain renders,
+           and p50/p99 latency under Poisson arrivals at a stated rate.
 
 Per-stage GPU time comes from HIP events the library records on the render
 stream around each stage, inside the timed region.  The roofline entry is the
 stage with the largest share; its algorithmic work per launch is:
   count      68 B / Gaussian (44 B geometry read, 8 B packed count written,
-             8 B read + 8 B written by the compaction scan)
-  emit       44 B / Gaussian + 248 B / visible splat (192 B SH read, 48 B
-             record + 8 B index/radius written) + 12 B / key
+             8 B read + 8 B written by the compaction scan) + 32 B / visible
+             splat (geometry cached for emit)
+  emit       24 B / Gaussian (position, opacity, scan offset) + 296 B / visible
+             splat (32 B cached geometry + 192 B SH read, 48 B record + 24 B
+             index, radius and key metadata written) + 12 B / key
   sort       8 B / key (one histogram read) + 24 B / key per digit pass
   ranges     8 B / key + 8 B / tile
   composite  24 FP32 operations per (pixel, splat) evaluation -- VALU bound;
              its bytes are 52 B / key + 8 B / tile + 16 B / pixel
+`traffic` is the HBM bytes per launch rocprofv3 counted for that kernel
+(FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy of
+known size), read from the committed summary under profiles/ when present.
 CPU baseline: the oracle (scalar C restatement of the reference path, one
-thread) rendering one frame of the same workload on this host.
+thread) rendering one frame of the headline workload on this host.
 """
 from __future__ import annotations
 
@@ -30,6 +50,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -41,6 +62,13 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 COMPOSITE_FLOPS_PER_EVAL = 24
 METRIC = "1M-splat 1080p frames/s (fwd, fwd+bwd); p50/p99 request latency"
+KERNELS = {"composite": "k_composite", "sort": "k_sort_hist+k_sort_scatter", "emit": "k_emit",
+           "count": "k_count", "ranges": "k_ranges"}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_traffic_random1m_1920x1080.md")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_model() -> str:
@@ -53,52 +81,63 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def stage_work(N, V, M, T, P, passes, E):
+def stage_work(N, V, M, T, passes, E):
     """Algorithmic bytes (or flops) per launch of each stage."""
     return {
-        "count": ("hbm", 68.0 * N),
-        "emit": ("hbm", 44.0 * N + 248.0 * V + 12.0 * M),
+        "count": ("hbm", 68.0 * N + 32.0 * V),
+        "emit": ("hbm", 24.0 * N + 296.0 * V + 12.0 * M),
         "sort": ("hbm", 8.0 * M + 24.0 * passes * M),
         "ranges": ("hbm", 8.0 * M + 8.0 * T),
         "composite": ("valu", float(COMPOSITE_FLOPS_PER_EVAL) * E),
     }
 
 
-def roofline_entry(bound, work, ms):
+def measured_traffic():
+    """kernel -> HBM bytes per launch from the committed rocprofv3 summary
+    (lines `| kernel | ... | traffic_bytes |`), or {}."""
+    out = {}
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                m = re.match(r"\|\s*(k_\w+)\s*\|.*\|\s*([0-9.]+)\s*\|\s*$", line)
+                if m:
+                    out[m.group(1)] = float(m.group(2))
+    except OSError:
+        pass
+    return out
+
+
+def roofline_entry(bound, work, ms, traffic=None):
     if bound == "hbm":
         achieved = work / (ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": int(work), "avg_ms": round(ms, 5)}
     achieved = work / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "algorithmic_flops": int(work), "avg_ms": round(ms, 5)}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="random1m_1920x1080")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-backward", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
-        return 2
+def timed(fn, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
+
+# ---- headline: one 1M / 1080p frame ----------------------------------------
+
+def headline(args, result):
     import numpy as np
     import torch
-    if not torch.cuda.is_available():
-        print("bench.py needs a HIP device", file=sys.stderr)
-        return 1
     from lumen_amd.raster import Rasterizer, RenderConfig
     from lumen_amd.scene import workload
 
-    print(f"[bench] generating {args.workload}", file=sys.stderr, flush=True)
+    log(f"generating {args.workload}")
     scene, cam = workload(args.workload)
     dev = scene.to_device("cuda:0")
     R = Rasterizer(0)
@@ -119,8 +158,7 @@ def main() -> int:
     for _ in range(args.warmup):
         R.render(dev, cam, cfg, out_rgb=rgb, out_depth=depth)
     torch.cuda.synchronize()
-    print(f"[bench] warmup done; timing {args.steps} frames", file=sys.stderr, flush=True)
-
+    log(f"warmup done; timing {args.steps} frames")
     R.set_profiling(True)
     acc = {}
     torch.cuda.synchronize()
@@ -135,53 +173,18 @@ def main() -> int:
     ms = (t1 - t0) * 1e3 / args.steps
     stage_ms = {k: v / args.steps for k, v in acc.items()}
 
-    work = stage_work(N, V, M, T, 0, passes, E)
-    stages = {k: roofline_entry(b, w, stage_ms[k]) for k, (b, w) in work.items()
-              if stage_ms.get(k, 0) > 0}
-
-    # forward + backward: L1 + D-SSIM against a render of a copy with means
-    # perturbed by N(0, 0.01), gradients for every parameter plane
-    fb = None
-    if not args.no_backward:
-        from lumen_amd.scene import perturbed
-        from lumen_amd.training import combined_loss, rasterize
-        with torch.no_grad():
-            target = R.render(perturbed(scene, 0.01, 7).to_device("cuda:0"), cam, cfg).rgb.clone()
-        planes = dev.planes.clone().requires_grad_(True)
-        shp = dev.sh.clone().requires_grad_(True)
-
-        def train_step():
-            planes.grad = None
-            shp.grad = None
-            img, _ = rasterize(planes, shp, cam, cfg, rasterizer=R)
-            loss = combined_loss(img, target)
-            loss.backward()
-            return loss
-
-        for _ in range(args.warmup):
-            train_step()
-        torch.cuda.synchronize()
-        R.set_profiling(True)
-        acc_b = {}
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            train_step()
-            for k, v in R.stage_times().items():
-                acc_b[k] = acc_b.get(k, 0.0) + v
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        R.set_profiling(False)
-        ms_fb = (t1 - t0) * 1e3 / args.steps
-        fb = {"value": round(1000.0 / ms_fb, 2), "unit": "fwd+bwd steps/s",
-              "ms_per_step": round(ms_fb, 4),
-              "loss": "0.8 L1 + 0.2 (1 - SSIM 8x8), target = render of means + N(0, 0.01)",
-              "stage_ms": {k: round(v / args.steps, 5) for k, v in acc_b.items()}}
+    traffic = measured_traffic()
+    work = stage_work(N, V, M, T, passes, E)
+    stages = {}
+    for k, (b, w) in work.items():
+        if stage_ms.get(k, 0) > 0:
+            kern = KERNELS[k]
+            tr = traffic.get(kern) if "+" not in kern else None
+            stages[k] = roofline_entry(b, w, stage_ms[k], int(tr) if tr else None)
     dominant = max(stages, key=lambda k: stages[k]["avg_ms"])
     roofline = dict(stages[dominant])
-    roofline["kernel"] = {"composite": "k_composite", "sort": "k_sort_hist+k_sort_scatter",
-                          "emit": "k_emit", "count": "k_count", "ranges": "k_ranges"}[dominant]
-
-    result = {
+    roofline["kernel"] = KERNELS[dominant]
+    result.update({
         "metric": METRIC,
         "value": round(1000.0 / ms, 2),
         "unit": "frames/s",
@@ -200,12 +203,261 @@ def main() -> int:
                    "sort_passes": passes},
         "roofline": roofline,
         "stages": stages,
-        "fwd_bwd": fb,
+    })
+
+    if "fwd_bwd" in args.legs:
+        from lumen_amd.scene import perturbed
+        from lumen_amd.training import combined_loss, rasterize
+        with torch.no_grad():
+            target = R.render(perturbed(scene, 0.01, 7).to_device("cuda:0"), cam, cfg).rgb.clone()
+        planes = dev.planes.clone().requires_grad_(True)
+        shp = dev.sh.clone().requires_grad_(True)
+
+        def train_step():
+            planes.grad = None
+            shp.grad = None
+            img, _ = rasterize(planes, shp, cam, cfg, rasterizer=R)
+            combined_loss(img, target).backward()
+
+        for _ in range(args.warmup):
+            train_step()
+        torch.cuda.synchronize()
+        R.set_profiling(True)
+        acc_b = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            train_step()
+            for k, v in R.stage_times().items():
+                acc_b[k] = acc_b.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        R.set_profiling(False)
+        ms_fb = (t1 - t0) * 1e3 / args.steps
+        result["fwd_bwd"] = {
+            "value": round(1000.0 / ms_fb, 2), "unit": "fwd+bwd steps/s",
+            "ms_per_step": round(ms_fb, 4),
+            "loss": "0.8 L1 + 0.2 (1 - SSIM 8x8), target = render of means + N(0, 0.01)",
+            "stage_ms": {k: round(v / args.steps, 5) for k, v in acc_b.items()}}
+        del planes, shp, target
+    R.close()
+    return scene, cam, cfg, V, M
+
+
+# ---- batch: 3M Gaussians x 32 orbit views -----------------------------------
+
+def batch_leg(args):
+    import torch
+    from lumen_amd.batch import MultiViewRenderer, autotune, rasterize_views
+    from lumen_amd.raster import RenderConfig
+    from lumen_amd.scene import orbit_workload, perturbed
+    from lumen_amd.training import combined_loss, rasterize
+
+    log("batch: generating 3M Gaussians, 32 orbit views")
+    scene, cams = orbit_workload(3_000_000, 32, 6.0)
+    dev = scene.to_device("cuda:0")
+    nv = len(cams)
+    mv = MultiViewRenderer(0)
+    log("batch: autotuning tile shape x LDS batch depth")
+    tune = autotune(mv, dev, cams, RenderConfig(), graph=True, reps=2)
+    cfg = tune.config
+    for _ in range(2):
+        out = mv.render(dev, cams, cfg, graph=True)
+    stats = mv.status(nv)
+    M = sum(s.total_splat_instances for s in stats)
+    V = sum(s.visible_splats for s in stats)
+    steps = max(3, args.steps // 6)
+    t_batch = timed(lambda: mv.render(dev, cams, cfg, graph=True), steps)
+    batched_rgb = out.rgb.clone()
+
+    R = mv.rasterizer
+    W, H = cams[0].config.width, cams[0].config.height
+    outs = [(torch.empty((H, W, 3), device="cuda:0"), torch.empty((H, W), device="cuda:0"))
+            for _ in range(nv)]
+
+    def plain():
+        for c, (o_rgb, o_d) in zip(cams, outs):
+            R.render(dev, c, cfg, out_rgb=o_rgb, out_depth=o_d)
+
+    plain()
+    t_plain = timed(plain, steps)
+    P = W * H
+    same = all(torch.equal(outs[k][0].reshape(-1), batched_rgb[3 * k * P: 3 * (k + 1) * P])
+               for k in range(nv))
+    res = {
+        "workload": "orbit3m_32x1920x1080", "views": nv, "gaussians": scene.count,
+        "visible": V, "instances": M,
+        "config": f"{cfg.tile_width}x{cfg.tile_height}/{cfg.batch_depth}",
+        "autotune_ms": tune.table,
+        "fwd_views_per_s": round(nv / t_batch, 2), "fwd_ms_per_frame": round(t_batch * 1e3, 3),
+        "plain_fwd_views_per_s": round(nv / t_plain, 2),
+        "fwd_speedup_vs_plain": round(t_plain / t_batch, 3),
+        "identical_to_plain": bool(same),
     }
+    del outs, batched_rgb
+    if "fwd_bwd" in args.legs:
+        log("batch: forward+backward")
+        with torch.no_grad():
+            tgt = mv.render(perturbed(scene, 0.01, 8).to_device("cuda:0"), cams, cfg).rgb.clone()
+        targets = [tgt[3 * k * P: 3 * (k + 1) * P].view(H, W, 3) for k in range(nv)]
+        planes = dev.planes.clone().requires_grad_(True)
+        shp = dev.sh.clone().requires_grad_(True)
+
+        def step_batched():
+            planes.grad = None
+            shp.grad = None
+            rgb, _ = rasterize_views(planes, shp, cams, cfg, mv, graph=True)
+            loss = sum(combined_loss(rgb[3 * k * P: 3 * (k + 1) * P].view(H, W, 3), targets[k])
+                       for k in range(nv)) / nv
+            loss.backward()
+
+        def step_plain():
+            planes.grad = None
+            shp.grad = None
+            for k, c in enumerate(cams):
+                img, _ = rasterize(planes, shp, c, cfg, rasterizer=R)
+                (combined_loss(img, targets[k]) / nv).backward()
+
+        fb_steps = max(2, args.steps // 10)
+        step_batched()
+        t_fb = timed(step_batched, fb_steps)
+        step_plain()
+        t_fb_plain = timed(step_plain, fb_steps)
+        res.update({"fwd_bwd_views_per_s": round(nv / t_fb, 2),
+                    "plain_fwd_bwd_views_per_s": round(nv / t_fb_plain, 2),
+                    "fwd_bwd_speedup_vs_plain": round(t_fb_plain / t_fb, 3)})
+        del planes, shp, tgt, targets
+    mv.close()
+    del dev, mv, R
+    torch.cuda.empty_cache()
+    return res
+
+
+# ---- serve: 8 resident scenes, mixed requests --------------------------------
+
+def serve_leg(args):
+    import torch
+    from lumen_amd.scene import SERVING_RESOLUTIONS, serving_scenes
+    from lumen_amd.serve import RenderServer, latency_summary, poisson_requests
+
+    log("serve: generating 8 resident scenes (14.61M Gaussians)")
+    scenes = {}
+    for name, sc in serving_scenes().items():
+        scenes[name] = sc.to_device("cuda:0")
+    names = list(scenes)
+    count = args.serve_requests
+
+    def target_fn(r):
+        H, W = r.camera.config.height, r.camera.config.width
+        g = torch.Generator(device="cuda:0").manual_seed(r.rid)
+        return torch.rand((H, W, 3), generator=g, device="cuda:0")
+
+    def make(seed, rate):
+        return poisson_requests(names, SERVING_RESOLUTIONS, rate=rate, count=count,
+                                backward_fraction=0.2, seed=seed)
+
+    def serve(server, reqs, realtime):
+        t0 = time.perf_counter()
+        done = server.run(reqs, realtime=realtime)
+        return done, time.perf_counter() - t0
+
+    # warm every scene and resolution once (allocations, first launches)
+    def server(slots, views):
+        return RenderServer(scenes, hbm_budget=budget, slots=slots, max_views=views,
+                            keep_outputs=False, target_fn=target_fn)
+
+    budget = 48 << 30
+    srv = server(3, 16)  # warm every scene and resolution once (allocations, first launches)
+    serve(srv, make(30, 1e9)[: 3 * len(names)], False)
+    log(f"serve: saturation, {count} requests, continuous batching")
+    frames0 = srv.frames
+    done, wall = serve(srv, make(31, 1e9), False)
+    sat = latency_summary(done, wall)
+    sat_frames = srv.frames - frames0
+    srv.close()
+    torch.cuda.empty_cache()
+    log("serve: saturation, one request at a time (plain per-view)")
+    srv = server(1, 1)
+    done_p, wall_p = serve(srv, make(31, 1e9), False)
+    sat_p = latency_summary(done_p, wall_p)
+    srv.close()
+    torch.cuda.empty_cache()
+    rate = round(0.7 * sat["throughput_rps"], 1)
+    log(f"serve: Poisson arrivals at {rate} req/s")
+    srv = server(3, 16)
+    done_r, wall_r = serve(srv, make(32, rate), True)
+    rt = latency_summary(done_r, wall_r)
+    srv.close()
+    res = {
+        "workload": "serve8_mixed", "scenes": {k: v.count for k, v in scenes.items()},
+        "requests": count, "fwd_bwd_fraction": 0.2,
+        "resolutions": [f"{w}x{h}" for w, h, _ in SERVING_RESOLUTIONS],
+        "saturation_rps": sat["throughput_rps"], "saturation_frames": sat_frames,
+        "plain_saturation_rps": sat_p["throughput_rps"],
+        "speedup_vs_plain": round(sat["throughput_rps"] / sat_p["throughput_rps"], 3),
+        "poisson_rate_rps": rate, "p50_ms": rt["p50_ms"], "p99_ms": rt["p99_ms"],
+        "mean_ms": rt["mean_ms"], "throughput_rps": rt["throughput_rps"],
+        "slots": 3, "hbm_budget_gib": budget >> 30,
+    }
+    del scenes
+    torch.cuda.empty_cache()
+    return res
+
+
+def copy_calibration():
+    """A plain device copy of known size (for scaling rocprofv3's FETCH_SIZE /
+    WRITE_SIZE counters): 256 MiB float32, 5 times."""
+    import torch
+    src = torch.rand(64 << 20, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "launches": 5}), flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="random1m_1920x1080")
+    ap.add_argument("--legs", default="fwd_bwd,batch,serve",
+                    help="comma list of fwd_bwd, batch, serve ('' for the headline only)")
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-backward", action="store_true", help="same as leaving fwd_bwd out")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    args.legs = {x.strip() for x in args.legs.split(",") if x.strip()}
+    if args.no_backward:
+        args.legs.discard("fwd_bwd")
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+
+    result = {}
+    scene, cam, cfg, V, M = headline(args, result)
+    for leg, key, fn in (("batch", "batched", batch_leg), ("serve", "serving", serve_leg)):
+        if leg in args.legs:
+            try:
+                result[key] = fn(args)
+            except Exception as e:  # a failing leg is reported, the line still prints
+                log(f"{leg} leg failed: {e!r}")
+                result[key] = {"error": repr(e)[:500]}
+            torch.cuda.empty_cache()
 
     if not args.no_cpu_baseline:
         from oracle import pyoracle
-        print("[bench] CPU baseline: oracle, one frame, one thread", file=sys.stderr, flush=True)
+        log("CPU baseline: oracle, one frame, one thread")
         try:
             os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
         except (AttributeError, OSError):

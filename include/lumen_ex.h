@@ -136,7 +136,8 @@ LumenStatus lumen_ex_frame_status(LumenExContext* ctx, LumenExViewStats* out_sta
 /* radix_sort_64bit (src/raster/radix_sort.mojo:39-87): stable ascending sort
  * of n 64-bit keys with 32-bit values, in place on device memory.  Digits
  * that are equal across all keys are skipped (a stable pass over them is the
- * identity). */
+ * identity).  Uses the context's frame buffers: the last frame's stages and
+ * its backward are no longer available afterwards. */
 LumenStatus lumen_ex_radix_sort(LumenExContext* ctx, uint64_t* keys, uint32_t* values,
                                 uint64_t n, void* stream);
 
@@ -207,9 +208,8 @@ LumenStatus lumen_ex_render_backward(LumenExContext* ctx, const LumenExScene* sc
 
 /* Backward of the last lumen_ex_render_batch (same cameras and config, with
  * LUMEN_EX_SAVE_PIXELS); grad_rgb/grad_depth in the batch's output layout.
- * Works after a graph replay too.  Each view's contribution is computed as
- * for one view; the contributions of a Gaussian seen in several views are
- * added atomically, so their order (only) is not fixed. */
+ * Works after a graph replay too.  One thread per Gaussian adds up its views'
+ * contributions in view order, so this too is the same on every run. */
 LumenStatus lumen_ex_render_batch_backward(LumenExContext* ctx, const LumenExScene* scene,
                                            const LumenExCamera* cameras, uint32_t n_views,
                                            const LumenExRenderConfig* config,

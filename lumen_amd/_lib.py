@@ -11,7 +11,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblumen.so")
+# the in-tree build; LUMEN_LIBRARY names another build of the same sources
+LIB_PATH = os.environ.get("LUMEN_LIBRARY") or os.path.join(_HERE, "liblumen.so")
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
 

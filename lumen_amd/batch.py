@@ -67,13 +67,26 @@ class MultiViewRenderer:
         self._outs: Dict[Tuple, Tuple[torch.Tensor, torch.Tensor]] = {}
         self.frame_id = 0
 
+    def close(self) -> None:
+        """Releases the context's device buffers and the output planes."""
+        self.stream.synchronize()
+        self._outs.clear()
+        self.rasterizer.close()
+
+    def _new_outputs(self, cameras) -> Tuple[torch.Tensor, torch.Tensor]:
+        P = frame_pixels(cameras)
+        with torch.cuda.stream(self.stream):
+            return (torch.empty(3 * P, dtype=torch.float32, device=f"cuda:{self.device}"),
+                    torch.empty(P, dtype=torch.float32, device=f"cuda:{self.device}"))
+
     def _outputs(self, cameras) -> Tuple[torch.Tensor, torch.Tensor]:
+        """The graph frame's output planes: graph replays need fixed pointers,
+        so they are kept (for the latest frame shape only) and overwritten by
+        the next frame."""
         key = tuple((c.config.width, c.config.height) for c in cameras)
         if key not in self._outs:
-            P = frame_pixels(cameras)
-            with torch.cuda.stream(self.stream):
-                self._outs[key] = (torch.empty(3 * P, dtype=torch.float32, device=f"cuda:{self.device}"),
-                                   torch.empty(P, dtype=torch.float32, device=f"cuda:{self.device}"))
+            self._outs.clear()
+            self._outs[key] = self._new_outputs(cameras)
         return self._outs[key]
 
     def render(self, scene: DeviceScene, cameras: Sequence[PinholeCamera],
@@ -87,7 +100,12 @@ class MultiViewRenderer:
         cams = (_lib.LumenExCamera * n)(*[camera_struct(c) for c in cameras])
         flags = (_lib.LUMEN_EX_GRAPH if graph else 0) | (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0)
         cfg = config.ex_struct(flags)
-        rgb, depth = out if out is not None else self._outputs(cameras)
+        if out is not None:
+            rgb, depth = out
+        elif graph:
+            rgb, depth = self._outputs(cameras)
+        else:
+            rgb, depth = self._new_outputs(cameras)
         sc = scene.ex_struct()
         self.stream.wait_stream(torch.cuda.current_stream(scene.device))
         stats = (_lib.LumenExViewStats * n)() if want_stats else None

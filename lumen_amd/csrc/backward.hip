@@ -23,11 +23,12 @@
 //   k_sum_partial     sums each visible splat's instance slots in a fixed
 //                     order (so the result does not depend on scheduling and
 //                     is the same run to run).
-//   k_preprocess_bwd  one thread per visible Gaussian: adjoints of the SH
-//                     colour (and view direction), the projection to screen,
-//                     the 2D inverse, the EWA projection, and the world
-//                     covariance R S S^T R^T with its quaternion
-//                     normalization; writes the Gaussian's gradient planes and
+//   k_preprocess_bwd  one thread per Gaussian, over every view it is visible
+//                     in: adjoints of the SH colour (and view direction), the
+//                     projection to screen, the 2D inverse and the EWA
+//                     projection per view, then of the world covariance
+//                     R S S^T R^T with its quaternion normalization once;
+//                     writes the Gaussian's gradient planes (no atomics) and
 //                     the view-space positional gradient norm for densification.
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -128,10 +129,11 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const uint32_t wtiles = (mt.z & 0xFFFFu) - min_tx + 1u;
       s_dst[j] = mt.x + ((uint32_t)ty - min_ty) * wtiles + ((uint32_t)tx - min_tx);
       const float4 a = rec[3u * vi + 0u];
+      const float4 b = rec[3u * vi + 1u];
       const float4 c = rec[3u * vi + 2u];
       s0[j] = a;
-      s1[j] = rec[3u * vi + 1u];
-      s2[j] = c;
+      s1[j] = b;
+      s2[j] = make_float4(c.x, c.y, power_limit(b.y), 0.0f);
       s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
                                       v.tile_w, rows, nwaves);
     }
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           power = 0.0f;
           pclamp = true;
         }
-        if (!(power > 4.5f)) {
+        const float4 c = s2[j];
+        if (!(power > c.z)) {  // power_limit: the forward skipped the others too
           const float G = splat_exp_neg(-power);
           float alpha = b.y * G;
           bool aclamp = false;
@@ -177,7 +180,6 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
           }
           if (!(alpha < kMinAlpha)) {
             contrib = true;
-            const float4 c = s2[j];
             const float om = 1.0f - alpha;
             const float rom = __builtin_amdgcn_rcpf(om);
             const float Tj = T * rom;
@@ -359,194 +361,202 @@ struct GradPlanes {
   int32_t* count;
 };
 
-// One view per frame: each Gaussian is written by one thread, plain adds.
-// Several views: a Gaussian visible in several views is written by several
-// threads, so the adds are atomic.
-__device__ __forceinline__ void acc(float* p, float x, bool atomic) {
-  if (atomic) atomicAdd(p, x);
-  else *p += x;
-}
-
+// One thread per Gaussian, looping over the frame's views: the per-view
+// parts (SH colour and view direction, projection, 2D inverse and EWA
+// adjoints) use that view's visible-splat gradient; their dL/dSigma (world
+// covariance) contributions are summed and the view-independent part
+// (Sigma = R S S^T R^T with its scale clamp and quaternion normalization)
+// runs once.  Each gradient plane entry is read-modified-written by exactly
+// one thread: no atomics, and the same result on every run for any number
+// of views.  The view's visible index comes from the scanned per-(view,
+// Gaussian) offsets (visible iff the next offset's visible count is larger).
 __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const ViewParams* __restrict__ views,
-                                                        const ViewSpan* __restrict__ spans,
                                                         int nviews,
+                                                        const uint64_t* __restrict__ offsets,
+                                                        const uint64_t* __restrict__ totals,
                                                         const SortPlan* __restrict__ plan,
-                                                        const int32_t* __restrict__ vis2orig,
                                                         const float* __restrict__ grad_vis,
-                                                        GradPlanes out, int atomic_adds) {
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  if (k >= plan->v || plan->overflow) return;
-  const bool atomic = atomic_adds != 0;
-  // owning view: the last one whose visible run starts at or before k
-  int lo = 0, hi = nviews - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (spans[mid].vis_base <= k) lo = mid;
-    else hi = mid - 1;
-  }
-  const ViewParams& v = views[lo];
-  const uint32_t i = (uint32_t)vis2orig[k];
-  const float* g = grad_vis + (size_t)k * kGradVis;
-  const float gmx = g[0], gmy = g[1], ga = g[2], gb = g[3], gc = g[4];
-  const float* W = v.rot;
-  float gpos[3] = {0.0f, 0.0f, 0.0f};
-
-  // opacity
-  acc(&out.p[10][i], g[5], atomic);
-
-  // colour = clamp(0.5 + sum_k sh_k Y_k(dir)), dir = normalize(pos - eye)
+                                                        GradPlanes out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= s.n || plan->overflow) return;
+  const uint64_t nv = (uint64_t)nviews * s.n;
   const float px = s.px[i], py = s.py[i], pz = s.pz[i];
-  float ux = px - v.eye[0], uy = py - v.eye[1], uz = pz - v.eye[2];
-  const float dl = sqrtf(ux * ux + uy * uy + uz * uz);
-  normalize3(ux, uy, uz);
-  float Y[16];
-  sh_basis(ux, uy, uz, Y);
-  const int ncoef = v.sh_coeffs;
+  const int ncoef = views[0].sh_coeffs;  // the same in every view of a frame
   const int nfloat = 3 * ncoef;
-  const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
-  float c[48];
+  bool loaded = false;
+  float c[48];     // SH coefficients (loaded at the first visible view)
+  float gsh[48];   // dL/dsh
 #pragma unroll
-  for (int qd = 0; qd < 12; ++qd) {
-    const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
-    c[4 * qd + 0] = x.x;
-    c[4 * qd + 1] = x.y;
-    c[4 * qd + 2] = x.z;
-    c[4 * qd + 3] = x.w;
+  for (int e = 0; e < 48; ++e) gsh[e] = 0.0f;
+  float gpos[3] = {0.0f, 0.0f, 0.0f};
+  float gop = 0.0f, vs = 0.0f;
+  int32_t seen = 0;
+  float S[3][3], M[9], R[9], sc[3];
+  bool sclamp[3];
+  float qn = 0.0f;
+  Quat qh{1.0f, 0.0f, 0.0f, 0.0f};
+  float F[3][3];  // dL/dSigma summed over the views (symmetric use)
+#pragma unroll
+  for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) F[rr][cc] = 0.0f;
+
+  for (int vw = 0; vw < nviews; ++vw) {
+    const uint64_t idx = (uint64_t)vw * s.n + i;
+    const uint64_t off0 = offsets[idx];
+    const uint64_t off1 = idx + 1 < nv ? offsets[idx + 1] : *totals;
+    if ((uint32_t)off1 == (uint32_t)off0) continue;  // not visible in this view
+    const uint32_t k = (uint32_t)off0;
+    const ViewParams& v = views[vw];
+    if (!loaded) {
+      loaded = true;
+      const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+#pragma unroll
+      for (int qd = 0; qd < 12; ++qd) {
+        const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+        c[4 * qd + 0] = x.x;
+        c[4 * qd + 1] = x.y;
+        c[4 * qd + 2] = x.z;
+        c[4 * qd + 3] = x.w;
+      }
+      // world covariance Sigma = M M^T, M = R(q) diag(s), as the forward
+      const Quat qraw{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
+      qn = sqrtf(quat_norm2(qraw));
+      qh = quat_normalized(qraw);
+      quat_matrix(qh, R);
+      sc[0] = s.sx[i];
+      sc[1] = s.sy[i];
+      sc[2] = s.sz[i];
+      for (int a = 0; a < 3; ++a) {
+        sclamp[a] = sc[a] < 0.00001f;
+        if (sclamp[a]) sc[a] = 0.00001f;
+      }
+      for (int rr = 0; rr < 3; ++rr)
+        for (int cc = 0; cc < 3; ++cc) M[3 * rr + cc] = R[3 * rr + cc] * sc[cc];
+      for (int rr = 0; rr < 3; ++rr)
+        for (int cc = 0; cc < 3; ++cc)
+          S[rr][cc] =
+              M[3 * rr] * M[3 * cc] + M[3 * rr + 1] * M[3 * cc + 1] + M[3 * rr + 2] * M[3 * cc + 2];
+    }
+    const float* g = grad_vis + (size_t)k * kGradVis;
+    const float gmx = g[0], gmy = g[1], ga = g[2], gb = g[3], gc = g[4];
+    const float* W = v.rot;
+    gop += g[5];
+
+    // colour = clamp(0.5 + sum_k sh_k Y_k(dir)), dir = normalize(pos - eye)
+    float ux = px - v.eye[0], uy = py - v.eye[1], uz = pz - v.eye[2];
+    const float dl = sqrtf(ux * ux + uy * uy + uz * uz);
+    normalize3(ux, uy, uz);
+    float Y[16];
+    sh_basis(ux, uy, uz, Y);
+    // the forward's colour sums, in its order, for the clamp decision
+    float raw[3] = {0.5f, 0.5f, 0.5f};
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+      if (kk < ncoef)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) raw[ch] += c[3 * kk + ch] * Y[kk];
+    float gcol[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) gcol[ch] = (raw[ch] < 0.0f || raw[ch] > 1.0f) ? 0.0f : g[6 + ch];
+    float dY[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      dY[kk] = 0.0f;
+      if (kk < ncoef) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          dY[kk] += gcol[ch] * c[3 * kk + ch];
+          gsh[3 * kk + ch] += gcol[ch] * Y[kk];
+        }
+      }
+    }
+    if (dl > 0.000001f) {
+      float gd[3];
+      sh_dir_grad(ux, uy, uz, dY, ncoef, gd);
+      const float dot = gd[0] * ux + gd[1] * uy + gd[2] * uz;
+      gpos[0] += (gd[0] - ux * dot) / dl;
+      gpos[1] += (gd[1] - uy * dot) / dl;
+      gpos[2] += (gd[2] - uz * dot) / dl;
+    }
+
+    // camera-space position (same rotation as the forward)
+    const Quat pq{0.0f, px - v.eye[0], py - v.eye[1], pz - v.eye[2]};
+    const Quat r = quat_mul(quat_mul(v.q_view, pq), v.q_view_inv);
+    const float x = r.x, y = r.y;
+    const float z = r.z < 0.0001f ? 0.0001f : r.z;
+    const float fx = v.fx, fy = v.fy;
+    float gpc[3];
+    gpc[0] = gmx * fx / z;
+    gpc[1] = gmy * fy / z;
+    gpc[2] = g[9] - gmx * fx * x / (z * z) - gmy * fy * y / (z * z);
+    vs += sqrtf(gmx * gmx + gmy * gmy);
+    ++seen;
+
+    // T = J W and the screen covariance
+    const float itz = 1.0f / z, itz2 = itz * itz;
+    const float j00 = fx * itz, j02 = -fx * x * itz2, j11 = fy * itz, j12 = -fy * y * itz2;
+    float t0[3], t1[3], St0[3], St1[3];
+    for (int a = 0; a < 3; ++a) {
+      t0[a] = j00 * W[a] + j02 * W[6 + a];
+      t1[a] = j11 * W[3 + a] + j12 * W[6 + a];
+    }
+    for (int rr = 0; rr < 3; ++rr) {
+      St0[rr] = S[rr][0] * t0[0] + S[rr][1] * t0[1] + S[rr][2] * t0[2];
+      St1[rr] = S[rr][0] * t1[0] + S[rr][1] * t1[1] + S[rr][2] * t1[2];
+    }
+    const float c00 = t0[0] * St0[0] + t0[1] * St0[1] + t0[2] * St0[2] + v.lowpass;
+    const float c01 = t0[0] * St1[0] + t0[1] * St1[1] + t0[2] * St1[2];
+    const float c11 = t1[0] * St1[0] + t1[1] * St1[1] + t1[2] * St1[2] + v.lowpass;
+    const float det = c00 * c11 - c01 * c01;
+    float gc00 = 0.0f, gc01 = 0.0f, gc11 = 0.0f;
+    if (!(det > -0.0000001f && det < 0.0000001f)) {
+      const float id = 1.0f / det;
+      const float K = (-ga * c11 + gb * c01 - gc * c00) * id * id;
+      gc00 = gc * id + K * c11;
+      gc11 = ga * id + K * c00;
+      gc01 = -gb * id - 2.0f * K * c01;
+    }
+    float gt0[3], gt1[3];
+    for (int a = 0; a < 3; ++a) {
+      gt0[a] = 2.0f * gc00 * St0[a] + gc01 * St1[a];
+      gt1[a] = gc01 * St0[a] + 2.0f * gc11 * St1[a];
+    }
+    for (int rr = 0; rr < 3; ++rr)
+      for (int cc = 0; cc < 3; ++cc)
+        F[rr][cc] += gc00 * t0[rr] * t0[cc] + 0.5f * gc01 * (t0[rr] * t1[cc] + t1[rr] * t0[cc]) +
+                     gc11 * t1[rr] * t1[cc];
+    const float gj00 = gt0[0] * W[0] + gt0[1] * W[1] + gt0[2] * W[2];
+    const float gj02 = gt0[0] * W[6] + gt0[1] * W[7] + gt0[2] * W[8];
+    const float gj11 = gt1[0] * W[3] + gt1[1] * W[4] + gt1[2] * W[5];
+    const float gj12 = gt1[0] * W[6] + gt1[1] * W[7] + gt1[2] * W[8];
+    gpc[2] += gj00 * (-fx * itz2) + gj02 * (2.0f * fx * x * itz2 * itz) + gj11 * (-fy * itz2) +
+              gj12 * (2.0f * fy * y * itz2 * itz);
+    gpc[0] += gj02 * (-fx * itz2);
+    gpc[1] += gj12 * (-fy * itz2);
+    for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
   }
-  // the forward's colour sums, in its order, for the clamp decision
-  float raw[3] = {0.5f, 0.5f, 0.5f};
-#pragma unroll
-  for (int kk = 0; kk < 16; ++kk)
-    if (kk < ncoef)
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) raw[ch] += c[3 * kk + ch] * Y[kk];
-  float gcol[3];
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch) gcol[ch] = (raw[ch] < 0.0f || raw[ch] > 1.0f) ? 0.0f : g[6 + ch];
-  float dY[16];
-#pragma unroll
-  for (int kk = 0; kk < 16; ++kk) {
-    dY[kk] = 0.0f;
-    if (kk < ncoef)
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) dY[kk] += gcol[ch] * c[3 * kk + ch];
-  }
-  // dL/dsh[k][ch] = gcol[ch] * Y[k], written as float4s (elements past the
-  // used coefficients are left as they are)
+  if (!seen) return;
+
+  out.p[10][i] += gop;
+  out.p[0][i] += gpos[0];
+  out.p[1][i] += gpos[1];
+  out.p[2][i] += gpos[2];
+  if (out.viewspace) out.viewspace[i] += vs;
+  if (out.count) out.count[i] += seen;
+  // dL/dsh as float4s (elements past the used coefficients are left as they are)
   float4* gsh4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
 #pragma unroll
   for (int qd = 0; qd < 12; ++qd) {
     if (4 * qd >= nfloat) continue;
-    float add[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int e = 4 * qd + m;
-      add[m] = gcol[e % 3] * Y[e / 3];
-    }
-    if (atomic) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if (4 * qd + m < nfloat) atomicAdd(reinterpret_cast<float*>(gsh4 + qd) + m, add[m]);
-    } else {
-      float4 o = gsh4[qd];
-      if (4 * qd + 0 < nfloat) o.x += add[0];
-      if (4 * qd + 1 < nfloat) o.y += add[1];
-      if (4 * qd + 2 < nfloat) o.z += add[2];
-      if (4 * qd + 3 < nfloat) o.w += add[3];
-      gsh4[qd] = o;
-    }
+    float4 o = gsh4[qd];
+    if (4 * qd + 0 < nfloat) o.x += gsh[4 * qd + 0];
+    if (4 * qd + 1 < nfloat) o.y += gsh[4 * qd + 1];
+    if (4 * qd + 2 < nfloat) o.z += gsh[4 * qd + 2];
+    if (4 * qd + 3 < nfloat) o.w += gsh[4 * qd + 3];
+    gsh4[qd] = o;
   }
-  if (dl > 0.000001f) {
-    float gd[3];
-    sh_dir_grad(ux, uy, uz, dY, ncoef, gd);
-    const float dot = gd[0] * ux + gd[1] * uy + gd[2] * uz;
-    gpos[0] += (gd[0] - ux * dot) / dl;
-    gpos[1] += (gd[1] - uy * dot) / dl;
-    gpos[2] += (gd[2] - uz * dot) / dl;
-  }
-
-  // camera-space position (same rotation as the forward)
-  const Quat pq{0.0f, px - v.eye[0], py - v.eye[1], pz - v.eye[2]};
-  const Quat r = quat_mul(quat_mul(v.q_view, pq), v.q_view_inv);
-  const float x = r.x, y = r.y;
-  const float z = r.z < 0.0001f ? 0.0001f : r.z;
-  const float fx = v.fx, fy = v.fy;
-  float gpc[3];
-  gpc[0] = gmx * fx / z;
-  gpc[1] = gmy * fy / z;
-  gpc[2] = g[9] - gmx * fx * x / (z * z) - gmy * fy * y / (z * z);
-  if (out.viewspace) acc(&out.viewspace[i], sqrtf(gmx * gmx + gmy * gmy), atomic);
-  if (out.count) {
-    if (atomic) atomicAdd(&out.count[i], 1);
-    else out.count[i] += 1;
-  }
-
-  // world covariance Sigma = M M^T, M = R(q) diag(s)
-  const Quat qraw{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
-  const float qn = sqrtf(quat_norm2(qraw));
-  const Quat qh = quat_normalized(qraw);
-  float R[9];
-  quat_matrix(qh, R);
-  float sc[3] = {s.sx[i], s.sy[i], s.sz[i]};
-  bool sclamp[3];
-  for (int a = 0; a < 3; ++a) {
-    sclamp[a] = sc[a] < 0.00001f;
-    if (sclamp[a]) sc[a] = 0.00001f;
-  }
-  float M[9];
-  for (int rr = 0; rr < 3; ++rr)
-    for (int cc = 0; cc < 3; ++cc) M[3 * rr + cc] = R[3 * rr + cc] * sc[cc];
-  float S[3][3];
-  for (int rr = 0; rr < 3; ++rr)
-    for (int cc = 0; cc < 3; ++cc)
-      S[rr][cc] = M[3 * rr] * M[3 * cc] + M[3 * rr + 1] * M[3 * cc + 1] + M[3 * rr + 2] * M[3 * cc + 2];
-
-  // T = J W and the screen covariance
-  const float itz = 1.0f / z, itz2 = itz * itz;
-  const float j00 = fx * itz, j02 = -fx * x * itz2, j11 = fy * itz, j12 = -fy * y * itz2;
-  float t0[3], t1[3], St0[3], St1[3];
-  for (int a = 0; a < 3; ++a) {
-    t0[a] = j00 * W[a] + j02 * W[6 + a];
-    t1[a] = j11 * W[3 + a] + j12 * W[6 + a];
-  }
-  for (int rr = 0; rr < 3; ++rr) {
-    St0[rr] = S[rr][0] * t0[0] + S[rr][1] * t0[1] + S[rr][2] * t0[2];
-    St1[rr] = S[rr][0] * t1[0] + S[rr][1] * t1[1] + S[rr][2] * t1[2];
-  }
-  const float c00 = t0[0] * St0[0] + t0[1] * St0[1] + t0[2] * St0[2] + v.lowpass;
-  const float c01 = t0[0] * St1[0] + t0[1] * St1[1] + t0[2] * St1[2];
-  const float c11 = t1[0] * St1[0] + t1[1] * St1[1] + t1[2] * St1[2] + v.lowpass;
-  const float det = c00 * c11 - c01 * c01;
-  float gc00 = 0.0f, gc01 = 0.0f, gc11 = 0.0f;
-  if (!(det > -0.0000001f && det < 0.0000001f)) {
-    const float id = 1.0f / det;
-    const float K = (-ga * c11 + gb * c01 - gc * c00) * id * id;
-    gc00 = gc * id + K * c11;
-    gc11 = ga * id + K * c00;
-    gc01 = -gb * id - 2.0f * K * c01;
-  }
-  float gt0[3], gt1[3], F[3][3];
-  for (int a = 0; a < 3; ++a) {
-    gt0[a] = 2.0f * gc00 * St0[a] + gc01 * St1[a];
-    gt1[a] = gc01 * St0[a] + 2.0f * gc11 * St1[a];
-  }
-  for (int rr = 0; rr < 3; ++rr)
-    for (int cc = 0; cc < 3; ++cc)
-      F[rr][cc] = gc00 * t0[rr] * t0[cc] + 0.5f * gc01 * (t0[rr] * t1[cc] + t1[rr] * t0[cc]) +
-                  gc11 * t1[rr] * t1[cc];
-  const float gj00 = gt0[0] * W[0] + gt0[1] * W[1] + gt0[2] * W[2];
-  const float gj02 = gt0[0] * W[6] + gt0[1] * W[7] + gt0[2] * W[8];
-  const float gj11 = gt1[0] * W[3] + gt1[1] * W[4] + gt1[2] * W[5];
-  const float gj12 = gt1[0] * W[6] + gt1[1] * W[7] + gt1[2] * W[8];
-  gpc[2] += gj00 * (-fx * itz2) + gj02 * (2.0f * fx * x * itz2 * itz) + gj11 * (-fy * itz2) +
-            gj12 * (2.0f * fy * y * itz2 * itz);
-  gpc[0] += gj02 * (-fx * itz2);
-  gpc[1] += gj12 * (-fy * itz2);
-  for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
-  acc(&out.p[0][i], gpos[0], atomic);
-  acc(&out.p[1][i], gpos[1], atomic);
-  acc(&out.p[2][i], gpos[2], atomic);
 
   // Sigma -> scale and rotation
   float gM[9], gR[9];
@@ -555,7 +565,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
       gM[3 * rr + cc] = 2.0f * (F[rr][0] * M[cc] + F[rr][1] * M[3 + cc] + F[rr][2] * M[6 + cc]);
   for (int cc = 0; cc < 3; ++cc) {
     const float gs = gM[cc] * R[cc] + gM[3 + cc] * R[3 + cc] + gM[6 + cc] * R[6 + cc];
-    if (!sclamp[cc]) acc(&out.p[3 + cc][i], gs, atomic);
+    if (!sclamp[cc]) out.p[3 + cc][i] += gs;
   }
   for (int rr = 0; rr < 3; ++rr)
     for (int cc = 0; cc < 3; ++cc) gR[3 * rr + cc] = gM[3 * rr + cc] * sc[cc];
@@ -571,7 +581,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
   if (qn > 0.000001f) {
     const float dot = gq[0] * w + gq[1] * qx + gq[2] * qy + gq[3] * qz;
     const float qq[4] = {w, qx, qy, qz};
-    for (int a = 0; a < 4; ++a) acc(&out.p[6 + a][i], (gq[a] - qq[a] * dot) / qn, atomic);
+    for (int a = 0; a < 4; ++a) out.p[6 + a][i] += (gq[a] - qq[a] * dot) / qn;
   }
 }
 
@@ -656,9 +666,9 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   hipLaunchKernelGGL(k_sum_partial, dim3(div_up(v_cap, 256)), dim3(256), 0, st, vmeta, part, plan,
                      gv);
   LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(v_cap, 256)), dim3(256), 0, st, scene_dev(sc),
-                     views, spans, nv, plan, c->vis2orig.as<int32_t>(), gv, gp,
-                     nv > 1 ? 1 : 0);
+  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(sc->count, 256)), dim3(256), 0, st,
+                     scene_dev(sc), views, nv, c->packed.as<uint64_t>(),
+                     c->totals.as<uint64_t>(), plan, gv, gp);
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, true, st));
   return LUMEN_SUCCESS;

@@ -465,10 +465,11 @@ __global__ __launch_bounds__(256) void k_composite(
       for (int j = t; j < n; j += nthreads) {
         const uint32_t vi = vals[base + j];
         const float4 a = rec[3u * vi + 0u];
+        const float4 b = rec[3u * vi + 1u];
         const float4 c = rec[3u * vi + 2u];
         s0[j] = a;
-        s1[j] = rec[3u * vi + 1u];
-        s2[j] = c;
+        s1[j] = b;
+        s2[j] = make_float4(c.x, c.y, power_limit(b.y), 0.0f);
         s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
                                         v.tile_w, rows, nwaves);
       }
@@ -495,12 +496,12 @@ __global__ __launch_bounds__(256) void k_composite(
           const float dy = pyc - a.y;
           float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
           if (power < 0.0f) power = 0.0f;
-          if (power > 4.5f) continue;
+          const float4 c = s2[j];
+          if (power > c.z) continue;  // power > 4.5, or alpha < 1/255 (power_limit)
           float alpha = b.y * splat_exp_neg(-power);
           if (alpha > 0.99f) alpha = 0.99f;
           // PixelAccumulator.blend, compositor.mojo:25-53
           if (alpha < kMinAlpha) continue;
-          const float4 c = s2[j];
           const float wgt = alpha * T;
           C0 += b.z * wgt;
           C1 += b.w * wgt;
@@ -1121,9 +1122,14 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
     LUMEN_HIP(hipMemcpyAsync(values, vb[1], n * 4, hipMemcpyDeviceToDevice, st));
   }
   LUMEN_HIP(hipStreamSynchronize(st));  // the staging above is reused by the next call
+  // the sort reused the frame's plan and ping-pong buffers: the last frame's
+  // stages and its backward are gone
   c->sizes_known = false;
   c->M = 0;
   c->V = 0;
+  c->kept = false;
+  c->pixels = false;
+  c->has_grad = false;
   return LUMEN_SUCCESS;
 }
 

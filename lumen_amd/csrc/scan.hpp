@@ -1,4 +1,5 @@
-// scan.hpp -- device-wide exclusive prefix sum (reduce / spine / downsweep).
+// scan.hpp -- device-wide exclusive prefix sum (reduce / spine / downsweep;
+// for up to kScanFuse chunks the downsweep adds up its predecessors itself).
 //
 // Used for the visible-splat compaction (packed visible flag + tile count per
 // Gaussian, 64-bit) and for the radix sort's digit-major block histograms
@@ -92,6 +93,68 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict_
   }
 }
 
+// k_scan_down for at most kScanFuse chunks: each block adds up the partial
+// sums of the chunks before it itself (no spine launch); the last block
+// writes the total.
+constexpr uint32_t kScanFuse = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __restrict__ in,
+                                                                  uint64_t n,
+                                                                  const T* __restrict__ partials,
+                                                                  T* __restrict__ out,
+                                                                  T* __restrict__ total_out) {
+  __shared__ T s[kScanChunk + kScanChunk / 32];
+  __shared__ T s_w[kScanThreads / 64];
+  __shared__ T s_prefix;
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  const int t = threadIdx.x;
+  {
+    T p = 0;
+    for (uint32_t b = t; b < blockIdx.x; b += kScanThreads) p += partials[b];
+    p = wave_sum(p);
+    if ((t & 63) == 0) s_w[t >> 6] = p;
+    __syncthreads();
+    if (t == 0) {
+      T q = 0;
+#pragma unroll
+      for (int w = 0; w < kScanThreads / 64; ++w) q += s_w[w];
+      s_prefix = q;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int li = j * kScanThreads + t;
+    const uint64_t idx = base + li;
+    s[scan_pad(li)] = idx < n ? in[idx] : T(0);
+  }
+  __syncthreads();
+  T v[kScanItems];
+  T sum = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    v[j] = s[scan_pad(t * kScanItems + j)];
+    sum += v[j];
+  }
+  T tot;
+  const T prefix = s_prefix;
+  T ex = block_exclusive_scan<T, kScanThreads>(sum, s_w, tot) + prefix;
+  if (t == 0 && total_out && blockIdx.x == gridDim.x - 1) *total_out = prefix + tot;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    s[scan_pad(t * kScanItems + j)] = ex;
+    ex += v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int li = j * kScanThreads + t;
+    const uint64_t idx = base + li;
+    if (idx < n) out[idx] = s[scan_pad(li)];
+  }
+}
+
 // Exclusive scan of n elements (in may equal out).  partials must hold
 // div_up(n, kScanChunk) elements; total_out (device, may be null) receives
 // the sum of all elements.
@@ -101,6 +164,12 @@ inline LumenStatus exclusive_scan(const T* in, T* out, uint64_t n, T* partials, 
   const uint32_t nb = div_up(n ? n : 1, kScanChunk);
   hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials);
   LUMEN_LAUNCHED();
+  if (nb <= kScanFuse) {
+    hipLaunchKernelGGL(k_scan_down_fused<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials,
+                       out, total_out);
+    LUMEN_LAUNCHED();
+    return LUMEN_SUCCESS;
+  }
   hipLaunchKernelGGL(k_scan_spine<T>, dim3(1), dim3(1024), 0, st, partials, nb, total_out);
   LUMEN_LAUNCHED();
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials, out);

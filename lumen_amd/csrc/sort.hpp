@@ -12,11 +12,14 @@
 //                   per-wave sub-histograms), stored digit-major so that one
 //                   exclusive scan gives every block's start offset per digit
 //   exclusive_scan  over the 256 x blocks table
-//   k_sort_scatter  stable rank inside the block: 256 consecutive elements
-//                   per round, a wave ranks its 64 lanes with 8 ballots
-//                   (lanes with equal digits are peers; rank = peers below
-//                   this lane), waves are ordered through LDS counters, so
-//                   equal digits keep their input order.
+//   k_sort_scatter  stable rank inside the block: each wave owns 1024
+//                   consecutive keys and ranks them 64 at a time with 8
+//                   ballots (lanes with equal digits are peers; rank = peers
+//                   below this lane) and wave-private digit counters in LDS;
+//                   one block scan then orders the waves, the block's keys are
+//                   staged in LDS in digit order and written out so that each
+//                   digit's run lands on consecutive addresses (coalesced).
+//                   Equal digits keep their input order.
 #pragma once
 
 #include "common.hpp"
@@ -64,8 +67,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(uint64_t* k0, uint64
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
     int pass, const uint32_t* __restrict__ offsets, uint32_t nb_cap) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[4][256];
+  __shared__ uint64_t s_key[kSortTile];
+  __shared__ uint32_t s_val[kSortTile];
+  __shared__ uint32_t s_cnt[4][256];  // per-wave digit counts, then per-wave starts
+  __shared__ uint32_t s_start[256];   // block-local start of each digit
+  __shared__ uint32_t s_glob[256];    // global start of each digit for this block
+  __shared__ uint32_t s_w[4];
   if (pass >= plan->passes) return;
   const uint32_t m = plan->m;
   if (blockIdx.x >= sort_blocks(m)) return;
@@ -76,20 +83,28 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const int shift = 8 * plan->digit[pass];
   const int t = threadIdx.x;
   const int w = t >> 6;
+  const int lane = t & 63;
   const uint64_t lt = lanemask_lt();
-  run[t] = offsets[(uint32_t)t * nb_cap + blockIdx.x];
-  const uint32_t base = blockIdx.x * kSortTile;
-  for (int j = 0; j < kSortItems; ++j) {
-    const uint32_t round = base + j * kSortThreads;
-    if (round >= m) break;  // uniform across the block
+  s_glob[t] = offsets[(uint32_t)t * nb_cap + blockIdx.x];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) wc[i][t] = 0;
-    __syncthreads();
-    const uint32_t idx = round + t;
-    const bool valid = idx < m;
-    const uint64_t k = valid ? kin[idx] : 0ull;
-    const uint32_t v = valid ? vin[idx] : 0u;
-    const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+  for (int i = 0; i < 4; ++i) s_cnt[i][t] = 0u;
+  __syncthreads();
+  // wave w owns keys [base + 1024 w, base + 1024 (w + 1)), 64 per round
+  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t wbase = base + (uint32_t)w * (kSortTile / 4);
+  uint64_t kk[kSortItems];
+  uint32_t vv[kSortItems];
+  uint32_t off[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const uint32_t idx = wbase + (uint32_t)j * 64u + (uint32_t)lane;
+    kk[j] = idx < m ? kin[idx] : 0ull;
+    vv[j] = idx < m ? vin[idx] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
+    const uint32_t d = (uint32_t)(kk[j] >> shift) & 0xFFu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -98,24 +113,47 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
       peers &= bit ? bb : ~bb;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      const uint32_t c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
-      const uint32_t r0 = run[t];
-      wc[0][t] = r0;
-      wc[1][t] = r0 + c0;
-      wc[2][t] = r0 + c0 + c1;
-      wc[3][t] = r0 + c0 + c1 + c2;
-      run[t] = r0 + c0 + c1 + c2 + c3;
+    const uint32_t before = valid ? s_cnt[w][d] : 0u;  // earlier rounds of this wave
+    off[j] = before + rank;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid && rank == 0) s_cnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __syncthreads();
+  {  // thread t: digit t's block-local start and each wave's start within it
+    const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+    uint32_t all;
+    const uint32_t st = block_exclusive_scan<uint32_t, kSortThreads>(c0 + c1 + c2 + c3, s_w, all);
+    s_start[t] = st;
+    s_cnt[0][t] = st;
+    s_cnt[1][t] = st + c0;
+    s_cnt[2][t] = st + c0 + c1;
+    s_cnt[3][t] = st + c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    if (wbase + (uint32_t)j * 64u + (uint32_t)lane < m) {
+      const uint32_t d = (uint32_t)(kk[j] >> shift) & 0xFFu;
+      const uint32_t pos = s_cnt[w][d] + off[j];
+      s_key[pos] = kk[j];
+      s_val[pos] = vv[j];
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t dst = wc[w][d] + rank;
-      kout[dst] = k;
-      vout[dst] = v;
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+  // the block's keys in digit order: each digit's run goes to consecutive
+  // addresses of the output
+  const uint32_t nblk = min((uint32_t)kSortTile, m - base);
+  for (uint32_t i = t; i < nblk; i += kSortThreads) {
+    const uint64_t key = s_key[i];
+    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+    const uint32_t dst = s_glob[d] + (i - s_start[d]);
+    kout[dst] = key;
+    vout[dst] = s_val[i];
   }
 }
 

@@ -328,6 +328,18 @@ LUMEN_HD float splat_exp_neg(float x) {
   return ldexpf(p, (int)k);
 }
 
+// The largest power at which alpha = op * exp(-power), as the compositor
+// computes it, can still reach 1/255: ln(255 op) plus a 1e-3 margin (four
+// orders of magnitude above the rounding of splat_exp, the product and
+// logf), capped at the reference's 4.5.  At a larger power alpha is below
+// 1/255 (the blend skips it) or the power test zeroes it, so the compositor
+// may skip the pixel before the exp with the same outcome.  NaN opacity
+// gives 4.5 (the pixel is evaluated as before).
+__device__ __forceinline__ float power_limit(float op) {
+  const float l = logf(255.0f * op) + 0.001f;
+  return l < 4.5f ? l : 4.5f;
+}
+
 // Bit w of the result: the splat's box (centre +- (hx, hy), outside of which
 // its power exceeds 4.5 at every pixel centre) meets the pixel centres of
 // wave w's strip of the tile (64 / tile_w full rows).  A wave whose bit is

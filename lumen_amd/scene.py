@@ -165,3 +165,24 @@ def workload(name: str):
         cfg = CameraConfig(1920, 1080, 1400.0, 1400.0, 960.0, 540.0, 0.01, 1000.0)
         return random_scene(1_000_000, math.sqrt(2.0), 1003), look_at(cfg, (0.0, 0.0, 5.0))
     raise KeyError(name)
+
+
+def orbit_workload(n: int = 3_000_000, views: int = 32, radius: float = 6.0):
+    """The multi-view workload: n Gaussians (the 1M workload's generator,
+    means N(0, 2 I)), `views` 1920x1080 cameras on a closed Catmull-Rom orbit
+    of the given radius looking at the origin."""
+    from .camera import CameraConfig, orbit
+    cfg = CameraConfig(1920, 1080, 1400.0, 1400.0, 960.0, 540.0, 0.01, 1000.0)
+    return random_scene(n, math.sqrt(2.0), 1004), orbit(cfg, views, radius)
+
+
+SERVING_SCENES = (10_000, 100_000, 500_000, 1_000_000, 2_000_000, 3_000_000, 3_000_000,
+                  5_000_000)
+SERVING_RESOLUTIONS = ((320, 240, 233.3), (1280, 720, 933.3), (1920, 1080, 1400.0),
+                       (3840, 2160, 2800.0))
+
+
+def serving_scenes(sizes=SERVING_SCENES):
+    """The serving workload's resident scenes (same generator, seeds 1100+k),
+    named s<k>_<n>."""
+    return {f"s{k}_{n}": random_scene(n, math.sqrt(2.0), 1100 + k) for k, n in enumerate(sizes)}

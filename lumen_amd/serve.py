@@ -72,25 +72,32 @@ class _Frame:
 
 def frame_memory_estimate(scene: DeviceScene, cams: Sequence[PinholeCamera],
                           backward: bool) -> int:
-    """Upper-bound style estimate of the device memory a frame touches
-    beyond the resident scene: per-(view, Gaussian) counts, records, keys
-    (8 B key + 4 B value, double buffered, ~4 tiles per visible splat),
-    outputs and, for a backward, gradient planes of the scene's size."""
+    """Upper-bound style estimate of the device memory a frame needs beyond
+    the resident scene, from the library's per-frame arrays: per (view,
+    Gaussian) 112 B (scan 8, geometry cache 32, record 48, meta 16, index and
+    radius 8) taking every Gaussian as visible; per key 24 B (double-buffered
+    8 B key + 4 B value) with keys per visible splat taken as
+    max(2, 8 (W / 1920)^2) (8 is what 1M Gaussians at 1080p measure); per
+    pixel 24 B of outputs and saved state.  A backward adds the gradient
+    planes (236 B per Gaussian), 40 B per key and 40 B per visible splat."""
     n = scene.count
     v = len(cams) * n
     pix = sum(c.config.width * c.config.height for c in cams)
-    m = 4 * v
-    mem = 8 * v + 56 * v + 24 * m + 24 * pix
+    m = sum(n * max(2.0, 8.0 * (c.config.width / 1920.0) ** 2) for c in cams)
+    mem = 112 * v + 24 * m + 24 * pix
     if backward:
-        mem += scene.nbytes() + 40 * v + 12 * pix
+        mem += 236 * n + 40 * m + 40 * v
     return int(mem)
 
 
 class RenderServer:
     def __init__(self, scenes: Dict[str, DeviceScene], hbm_budget: int = 32 << 30,
                  slots: int = 3, max_views: int = 16, config: Optional[RenderConfig] = None,
-                 device: int = 0):
+                 device: int = 0, keep_outputs: bool = True,
+                 target_fn: Optional[Callable[[Request], torch.Tensor]] = None):
         self.scenes = scenes
+        self.target_fn = target_fn  # makes a fwd+bwd request's target when it has none
+        self.keep_outputs = keep_outputs  # False: drop images/gradients once a request is done
         self.budget = int(hbm_budget)
         self.config = config or RenderConfig()
         self.max_views = max_views
@@ -110,14 +117,26 @@ class RenderServer:
 
     def _pick_frame(self) -> Optional[List[Request]]:
         """The oldest waiting request and, if it is a forward request, the other
-        waiting forward requests of its scene (arrival order)."""
+        waiting forward requests of its scene (arrival order) while the frame's
+        memory estimate stays within one slot's share of the budget (a slot's
+        buffers keep the size of the largest frame it ran)."""
         if not self.queue:
             return None
         head = self.queue[0]
         if head.backward:
             return [head]
-        group = [r for r in self.queue if not r.backward and r.scene == head.scene]
-        return group[: self.max_views]
+        scene = self.scenes[head.scene]
+        cap = self.budget // len(self.slots)
+        group = [head]
+        for r in self.queue:
+            if len(group) >= self.max_views:
+                break
+            if r is head or r.backward or r.scene != head.scene:
+                continue
+            if frame_memory_estimate(scene, [g.camera for g in group] + [r.camera], False) > cap:
+                continue
+            group.append(r)
+        return group
 
     def _admit(self) -> bool:
         if not self.free:
@@ -150,6 +169,10 @@ class RenderServer:
             from .batch import rasterize_views
             from .training import combined_loss
             with torch.cuda.stream(R.stream):
+                if r.target is None:
+                    if self.target_fn is None:
+                        raise ValueError(f"request {r.rid}: fwd+bwd without a target")
+                    r.target = self.target_fn(r)
                 planes = scene.planes.detach().requires_grad_(True)
                 sh = scene.sh.detach().requires_grad_(True)
                 rgb, depth = rasterize_views(planes, sh, cams, self.config, R)
@@ -163,18 +186,11 @@ class RenderServer:
                 ev = torch.cuda.Event()
                 ev.record(R.stream)
             return _Frame(slot, group, 0, ev)
-        out = R.render(scene, cams, self.config)
+        out = R.render(scene, cams, self.config)  # outputs allocated for this frame
         ev = torch.cuda.Event()
         ev.record(R.stream)
         for r, rgb, dep in zip(group, out.views_rgb, out.views_depth):
             r.rgb, r.depth = rgb, dep
-        # the outputs are the slot's reusable planes: copy them out on the slot's
-        # stream before the slot can be reused
-        with torch.cuda.stream(R.stream):
-            for r in group:
-                r.rgb = r.rgb.clone()
-                r.depth = r.depth.clone()
-            ev.record(R.stream)
         return _Frame(slot, group, 0, ev)
 
     def _retire(self) -> int:
@@ -186,12 +202,20 @@ class RenderServer:
                     r.done = now
                     if r.loss is not None and isinstance(r.loss, torch.Tensor):
                         r.loss = float(r.loss)
+                    if not self.keep_outputs:
+                        r.rgb = r.depth = r.grad_planes = r.grad_sh = r.target = None
                     self.completed.append(r)
                 self.inflight.remove(f)
                 self.mem_in_use -= f.mem
                 self.free.append(f.slot)
                 n += 1
         return n
+
+    def close(self) -> None:
+        """Releases every slot's device buffers."""
+        for R in self.slots:
+            R.close()
+        self.slots = []
 
     # -- driving --------------------------------------------------------------
     def run(self, requests: Sequence[Request], realtime: bool = True,

@@ -478,3 +478,23 @@ def test_psnr_scaling(oracle):
 def test_ssim_identical(oracle):
     a = np.full(16 * 16 * 3, 0.5, np.float32)
     assert oracle.lib().orc_ssim(oracle.fptr(a), oracle.fptr(a.copy()), 16, 16, 8) == pytest.approx(1.0, abs=1e-4)
+
+
+def test_power_limit_is_conservative(oracle):
+    """The compositors skip a pixel before the exp when power > min(4.5,
+    ln(255 op) + 1e-3): there the reference's alpha = op * exp(-power) is below
+    1/255 (or the power test applies), so the skip changes nothing.  Checked
+    with the oracle's exp at the first float above the limit and beyond."""
+    rng = np.random.default_rng(77)
+    op = np.concatenate([rng.uniform(1.0 / 255.0, 1.0, 200_000),
+                         np.float32(1.0 / 255.0) * (1 + rng.uniform(0, 1e-3, 20_000)),
+                         rng.uniform(0.34, 0.36, 20_000)]).astype(np.float32)
+    lim = (np.log(np.float32(255.0) * op).astype(np.float32) + np.float32(0.001)).astype(np.float32)
+    checked = 0
+    for step in (0.0, 1e-6, 1e-4, 1e-2, 0.5):
+        p = np.nextafter((lim + np.float32(step)).astype(np.float32), np.float32(np.inf))
+        keep = p <= np.float32(4.5)  # beyond 4.5 the power test itself skips
+        alpha = (op[keep] * oracle.expf_array(-p[keep].astype(np.float32))).astype(np.float32)
+        assert np.all(alpha < np.float32(0.00392156862)), step
+        checked += int(keep.sum())
+    assert checked > 100_000
