@@ -74,7 +74,8 @@ typedef struct {
     int32_t white_background;
     int32_t compute_depth;
     uint32_t flags;          /* LUMEN_EX_* below */
-    uint32_t batch_depth;    /* compositor LDS batch: 0 (= 256), 64, 128 or 256 */
+    uint32_t batch_depth;    /* compositor LDS batch: 64, 128 or 256; 0 = 128 for tiles of a
+                                multiple of 128 pixels (two pixels per lane), else 256 */
 } LumenExRenderConfig;
 
 #define LUMEN_EX_KEEP_STAGES 1u   /* keep the unsorted keys/values for lumen_ex_copy_stage */
@@ -215,6 +216,18 @@ LumenStatus lumen_ex_render_batch_backward(LumenExContext* ctx, const LumenExSce
                                            const LumenExRenderConfig* config,
                                            const float* grad_rgb, const float* grad_depth,
                                            LumenExGradients* grads, void* stream);
+
+/* The training step's photometric loss on device images (H*W*3 floats,
+ * interleaved RGB): compute_l1_loss, compute_ssim_window (8x8
+ * non-overlapping windows, channels pooled) and compute_combined_loss
+ * (src/optim/loss.mojo:11-35, 65-133, 136-144).  out (device, 3 floats)
+ * receives L1, SSIM and (1 - lambda) L1 + lambda (1 - SSIM); grad (device,
+ * H*W*3, or NULL) receives d(loss)/d(pred) with the reference's L1
+ * subgradient (+1/N where pred >= target, -1/N elsewhere).  Partial sums
+ * are added in a fixed order (the same result on every run). */
+LumenStatus lumen_ex_photometric_loss(LumenExContext* ctx, const float* pred, const float* target,
+                                      uint32_t width, uint32_t height, float lambda_dssim,
+                                      float* out, float* grad, void* stream);
 
 /* Per-stage GPU timing with HIP events recorded on the render's stream around
  * each stage of lumen_ex_render (off by default).  lumen_ex_stage_times waits

@@ -177,6 +177,10 @@ _SIGS = {
                                                       ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.POINTER(LumenExGradients),
                                                       ctypes.c_void_p]),
+    "lumen_ex_photometric_loss": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                                 ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
     "lumen_ex_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lumen_ex_stage_times": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
 }

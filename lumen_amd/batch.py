@@ -163,25 +163,32 @@ class RasterizeViews(torch.autograd.Function):
             depth = torch.empty(P, dtype=torch.float32, device=planes.device)
         renderer.render(scene, cameras, config, graph=graph, save_pixels=True, out=(rgb, depth))
         torch.cuda.current_stream(planes.device).wait_stream(renderer.stream)
+        if graph:
+            # fresh tensor objects over the renderer's fixed planes, so that the
+            # next frame's outputs are other autograd outputs
+            rgb, depth = rgb.view(-1), depth.view(-1)
         ctx.scene, ctx.cameras, ctx.config, ctx.renderer = scene, cameras, config, renderer
         ctx.viewspace = viewspace
         ctx.frame = renderer.rasterizer.frame_id
         ctx.graph = graph
-        ctx.out = (rgb, depth)
+        # outputs go through save_for_backward: kept on ctx directly they would
+        # form a cycle (output -> grad_fn -> ctx -> output) that is never freed
+        ctx.save_for_backward(rgb, depth)
         return rgb, depth
 
     @staticmethod
     def backward(ctx, grad_rgb, grad_depth):
         from .training import GaussianGradients
         R = ctx.renderer
+        out_rgb, out_depth = ctx.saved_tensors
         if R.rasterizer.frame_id != ctx.frame:
             R.render(ctx.scene, ctx.cameras, ctx.config, graph=ctx.graph, save_pixels=True,
-                     out=ctx.out)
+                     out=(out_rgb, out_depth))
         grads = GaussianGradients(ctx.scene.count, ctx.scene.device)
         if ctx.viewspace is not None:
             grads.viewspace_grad_2d = ctx.viewspace.viewspace_grad_2d
             grads.observation_count = ctx.viewspace.observation_count
-        g_rgb = grad_rgb.contiguous() if grad_rgb is not None else torch.zeros_like(ctx.out[0])
+        g_rgb = grad_rgb.contiguous() if grad_rgb is not None else torch.zeros_like(out_rgb)
         g_d = grad_depth.contiguous() if grad_depth is not None else None
         R.backward(ctx.scene, ctx.cameras, ctx.config, g_rgb, g_d, grads)
         return grads.planes, grads.sh, None, None, None, None, None

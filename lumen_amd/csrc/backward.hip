@@ -60,7 +60,7 @@ __global__ void k_zero_partial(float* __restrict__ g, const SortPlan* __restrict
     g[i] = 0.0f;
 }
 
-template <int BATCH>
+template <int BATCH, int PIX>
 __global__ __launch_bounds__(256) void k_composite_bwd(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
@@ -73,17 +73,17 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint32_t s_dst[BATCH];  // emission-order slot of each staged instance
   __shared__ uint8_t s_mask[BATCH];
-  __shared__ uint8_t s_list[4][BATCH];
-  __shared__ float s_part[4][BATCH * kGradVis];  // per-wave sums, one slot per splat
+  __shared__ uint8_t s_list[4 / PIX][BATCH];  // one list per wave
+  __shared__ float s_part[4 / PIX][BATCH * kGradVis];  // per-wave sums, one slot per splat
   __shared__ int s_last;
   if (plan->overflow) return;
   const uint32_t gt = blockIdx.x;
   const int vw = view_of_tile_b(views, nviews, gt);
   const ViewParams& v = views[vw];
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
-  const int nthreads = v.tile_w * v.tile_h;
+  const int nthreads = v.tile_w * v.tile_h / PIX;
   const int nwaves = nthreads >> 6;
-  const int rows = 64 / v.tile_w;
+  const int rows = 64 * PIX / v.tile_w;
   const int tile = (int)(gt - v.tile_base);
   const int2 rg = ranges[gt];
   if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
@@ -93,32 +93,40 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const int lane = t & 63;
   const int wv = t >> 6;
   const int px = tx * v.tile_w + t % v.tile_w;
-  const int py = ty * v.tile_h + t / v.tile_w;
-  const bool inside = px < v.width && py < v.height;
-  for (int e = t; e < 4 * BATCH * kGradVis; e += nthreads) (&s_part[0][0])[e] = 0.0f;
+  const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
+  for (int e = t; e < (4 / PIX) * BATCH * kGradVis; e += nthreads) (&s_part[0][0])[e] = 0.0f;
   if (t == 0) s_last = rg.x - 1;
   __syncthreads();
-  int last = rg.x - 1;
-  float T = 1.0f, gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gD = 0.0f;
-  if (inside) {
-    const size_t pix = v.pix_base + (size_t)py * v.width + px;
-    last = rg.x + n_eval[pix] - 1;
-    T = t_final[pix];
-    gC0 = g_rgb[3 * pix + 0];
-    gC1 = g_rgb[3 * pix + 1];
-    gC2 = g_rgb[3 * pix + 2];
-    gD = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
+  int last[PIX];
+  float T[PIX], gC0[PIX], gC1[PIX], gC2[PIX], gD[PIX], Tf[PIX], gsum[PIX];
+  float SC0[PIX], SC1[PIX], SC2[PIX], SD[PIX], pyc[PIX];
+  int lane_last = rg.x - 1;
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    last[p] = rg.x - 1;
+    T[p] = 1.0f;
+    gC0[p] = gC1[p] = gC2[p] = gD[p] = 0.0f;
+    SC0[p] = SC1[p] = SC2[p] = SD[p] = 0.0f;
+    pyc[p] = (float)(py0 + p) + 0.5f;
+    if (px < v.width && py0 + p < v.height) {
+      const size_t pix = v.pix_base + (size_t)(py0 + p) * v.width + px;
+      last[p] = rg.x + n_eval[pix] - 1;
+      T[p] = t_final[pix];
+      gC0[p] = g_rgb[3 * pix + 0];
+      gC1[p] = g_rgb[3 * pix + 1];
+      gC2[p] = g_rgb[3 * pix + 2];
+      gD[p] = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
+    }
+    Tf[p] = T[p];
+    gsum[p] = gC0[p] + gC1[p] + gC2[p];
+    lane_last = max(lane_last, last[p]);
   }
   // splats this wave's pixels evaluated: positions [0, wave_cnt) of the list
-  const uint32_t wave_cnt = wave_max_u32_all((uint32_t)(last - rg.x + 1));
+  const uint32_t wave_cnt = wave_max_u32_all((uint32_t)(lane_last - rg.x + 1));
   if (lane == 0) atomicMax(&s_last, rg.x + (int)wave_cnt - 1);
   __syncthreads();
   const int block_last = s_last;
-  const float Tf = T;
-  const float gsum = gC0 + gC1 + gC2;
   const float pxc = (float)px + 0.5f;
-  const float pyc = (float)py + 0.5f;
-  float SC0 = 0.0f, SC1 = 0.0f, SC2 = 0.0f, SD = 0.0f;
   for (int end = block_last + 1; end > rg.x; end -= BATCH) {
     const int start = max(rg.x, end - BATCH);
     const int n = end - start;
@@ -131,9 +139,9 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const float4 a = rec[3u * vi + 0u];
       const float4 b = rec[3u * vi + 1u];
       const float4 c = rec[3u * vi + 2u];
-      s0[j] = a;
-      s1[j] = b;
-      s2[j] = make_float4(c.x, c.y, power_limit(b.y), 0.0f);
+      s0[j] = a;                                              // x, y, i00, 2 i01
+      s1[j] = make_float4(b.x, b.y, power_limit(b.y), 0.0f);  // i11, opacity, power limit
+      s2[j] = make_float4(b.z, b.w, c.x, c.y);                // r, g, b, depth
       s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
                                       v.tile_w, rows, nwaves);
     }
@@ -154,60 +162,62 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int q = (int)cnt - 1; q >= 0; --q) {  // back to front; every lane runs every q
       const int j = s_list[wv][q];
+      const float4 a = s0[j];  // x, y, i00, 2 i01
+      const float4 b = s1[j];  // i11, opacity, power limit
+      const float4 c = s2[j];  // r, g, b, depth
+      const float dx = pxc - a.x;
+      const float txx = a.z * dx * dx;
+      const float txy = a.w * dx;
       float gv[kGradVis];
 #pragma unroll
       for (int k = 0; k < kGradVis; ++k) gv[k] = 0.0f;
       bool contrib = false;
-      if (start + j <= last) {
-        const float4 a = s0[j];
-        const float4 b = s1[j];
-        const float dx = pxc - a.x;
-        const float dy = pyc - a.y;
-        float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
+#pragma unroll
+      for (int p = 0; p < PIX; ++p) {
+        if (start + j > last[p]) continue;
+        const float dy = pyc[p] - a.y;
+        float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
         bool pclamp = false;
         if (power < 0.0f) {
           power = 0.0f;
           pclamp = true;
         }
-        const float4 c = s2[j];
-        if (!(power > c.z)) {  // power_limit: the forward skipped the others too
-          const float G = splat_exp_neg(-power);
-          float alpha = b.y * G;
-          bool aclamp = false;
-          if (alpha > 0.99f) {
-            alpha = 0.99f;
-            aclamp = true;
-          }
-          if (!(alpha < kMinAlpha)) {
-            contrib = true;
-            const float om = 1.0f - alpha;
-            const float rom = __builtin_amdgcn_rcpf(om);
-            const float Tj = T * rom;
-            const float w = alpha * Tj;
-            gv[6] = gC0 * w;
-            gv[7] = gC1 * w;
-            gv[8] = gC2 * w;
-            gv[9] = gD * w;
-            float dal = gC0 * (b.z * Tj - SC0 * rom) + gC1 * (b.w * Tj - SC1 * rom) +
-                        gC2 * (c.x * Tj - SC2 * rom) + gD * (c.y * Tj - SD * rom);
-            if (white) dal -= gsum * Tf * rom;
-            SC0 += b.z * w;
-            SC1 += b.w * w;
-            SC2 += c.x * w;
-            SD += c.y * w;
-            T = Tj;
-            if (!aclamp) {
-              gv[5] = dal * G;
-              if (!pclamp) {
-                const float dp = -dal * b.y * G;
-                const float i01 = 0.5f * a.w;
-                gv[2] = dp * 0.5f * dx * dx;
-                gv[3] = dp * dx * dy;
-                gv[4] = dp * 0.5f * dy * dy;
-                gv[0] = -dp * (a.z * dx + i01 * dy);
-                gv[1] = -dp * (i01 * dx + b.x * dy);
-              }
-            }
+        if (power > b.z) continue;  // power_limit: the forward skipped it too
+        const float G = splat_exp_neg(-power);
+        float alpha = b.y * G;
+        bool aclamp = false;
+        if (alpha > 0.99f) {
+          alpha = 0.99f;
+          aclamp = true;
+        }
+        if (alpha < kMinAlpha) continue;
+        contrib = true;
+        const float om = 1.0f - alpha;
+        const float rom = __builtin_amdgcn_rcpf(om);
+        const float Tj = T[p] * rom;
+        const float w = alpha * Tj;
+        gv[6] += gC0[p] * w;
+        gv[7] += gC1[p] * w;
+        gv[8] += gC2[p] * w;
+        gv[9] += gD[p] * w;
+        float dal = gC0[p] * (c.x * Tj - SC0[p] * rom) + gC1[p] * (c.y * Tj - SC1[p] * rom) +
+                    gC2[p] * (c.z * Tj - SC2[p] * rom) + gD[p] * (c.w * Tj - SD[p] * rom);
+        if (white) dal -= gsum[p] * Tf[p] * rom;
+        SC0[p] += c.x * w;
+        SC1[p] += c.y * w;
+        SC2[p] += c.z * w;
+        SD[p] += c.w * w;
+        T[p] = Tj;
+        if (!aclamp) {
+          gv[5] += dal * G;
+          if (!pclamp) {
+            const float dp = -dal * b.y * G;
+            const float i01 = 0.5f * a.w;
+            gv[2] += dp * 0.5f * dx * dx;
+            gv[3] += dp * dx * dy;
+            gv[4] += dp * 0.5f * dy * dy;
+            gv[0] += -dp * (a.z * dx + i01 * dy);
+            gv[1] += -dp * (i01 * dx + b.x * dy);
           }
         }
       }
@@ -643,18 +653,23 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   const float* tf = c->t_final.as<float>();
   const int32_t* ne = c->n_eval.as<int32_t>();
   const int nv = (int)c->nviews;
-  const dim3 grid((uint32_t)c->T), block(threads);
+  // one pixel per lane: the reverse pass keeps four sums and the adjoint
+  // terms per pixel, and two pixels per lane measured no faster (fewer waves
+  // resident)
+  const bool two = false;
+  const dim3 grid((uint32_t)c->T), block(two ? threads / 2 : threads);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
-  switch (batch) {
-    case 64:
-      hipLaunchKernelGGL(k_composite_bwd<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
-      break;
-    default:
-      hipLaunchKernelGGL(k_composite_bwd<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
-      break;
+#define LUMEN_COMPOSITE_BWD(B, P)                                                             \
+  hipLaunchKernelGGL((k_composite_bwd<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec,  \
+                     views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part)
+  if (two) {
+    if (batch == 64) LUMEN_COMPOSITE_BWD(64, 2);
+    else LUMEN_COMPOSITE_BWD(128, 2);
+  } else {
+    if (batch == 64) LUMEN_COMPOSITE_BWD(64, 1);
+    else LUMEN_COMPOSITE_BWD(128, 1);
   }
+#undef LUMEN_COMPOSITE_BWD
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, true, st));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, false, st));

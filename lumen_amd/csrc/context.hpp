@@ -79,6 +79,7 @@ struct LumenExContext {
   lumen::DevBuf ranges, t_final, n_eval, minmax;
   lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
+  lumen::DevBuf loss_part; // lumen_ex_photometric_loss: per-block partial sums
   void* pinned = nullptr;       // small host staging (totals, bounds, plan)
   void* pinned_views = nullptr; // host staging of the per-view parameters
   size_t pinned_views_cap = 0;

@@ -417,15 +417,18 @@ __device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views
   return lo;
 }
 
-// One workgroup per tile, one lane per pixel (lane t -> (t % tile_w, t /
-// tile_w), so a wave covers 64/tile_w full rows).  Splat records of the
-// tile's sorted range are staged through LDS BATCH at a time with the strip
-// mask of each; every wave then compacts the batch to the splats whose box
-// meets its strip (ballot + prefix count, order kept) and each lane walks
-// that list front to back exactly as PixelAccumulator does
-// (src/raster/compositor.mojo:25-89).  A lane stops at T <= 1e-4; the block
-// stops when all its lanes have.
-template <int BATCH>
+// One workgroup per tile; lane t owns PIX vertically adjacent pixels
+// (column t % tile_w, rows (t / tile_w) PIX + p), so a wave covers 64 PIX /
+// tile_w full rows.  Splat records of the tile's sorted range are staged
+// through LDS BATCH at a time with the strip mask of each (and the power
+// limit past which alpha < 1/255); every wave then compacts the batch to the
+// splats whose box meets its strip (ballot + prefix count, order kept), and
+// each pixel walks that list front to back exactly as PixelAccumulator does
+// (src/raster/compositor.mojo:25-89).  A lane's pixels share the splat's
+// loads and the x terms of the power, i00 dx dx and 2 i01 dx (each pixel's
+// power keeps the reference's expression tree).  A pixel stops at
+// T <= 1e-4, a lane when both of its pixels have, the block when all lanes have.
+template <int BATCH, int PIX>
 __global__ __launch_bounds__(256) void k_composite(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
@@ -435,41 +438,51 @@ __global__ __launch_bounds__(256) void k_composite(
   static_assert(BATCH <= 256, "list entries are bytes");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint8_t s_mask[BATCH];
-  __shared__ uint8_t s_list[4][BATCH];
+  __shared__ uint8_t s_list[4 / PIX][BATCH];  // one list per wave
   const uint32_t gt = blockIdx.x;
   const int vw = view_of_tile(views, nviews, gt);
   const ViewParams& v = views[vw];
-  const int nthreads = v.tile_w * v.tile_h;
+  const int nthreads = v.tile_w * v.tile_h / PIX;
   const int nwaves = nthreads >> 6;
-  const int rows = 64 / v.tile_w;
+  const int rows = 64 * PIX / v.tile_w;
   const int tile = (int)(gt - v.tile_base);
   const int tx = tile % v.tiles_x, ty = tile / v.tiles_x;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
   const int px = tx * v.tile_w + t % v.tile_w;
-  const int py = ty * v.tile_h + t / v.tile_w;
-  const bool inside = px < v.width && py < v.height;
+  const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
-  float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, T = 1.0f, D = 0.0f;
-  // evaluations = position of the terminating splat + 1, or the list length
-  int32_t ne = rg.y - rg.x;
+  float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
+  int32_t ne[PIX];  // position of the terminating splat + 1, or the list length
+  bool live[PIX];
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    C0[p] = C1[p] = C2[p] = D[p] = 0.0f;
+    T[p] = 1.0f;
+    ne[p] = rg.y - rg.x;
+    live[p] = px < v.width && py0 + p < v.height;
+  }
   if (rg.x != -1 && rg.x < rg.y) {
     const float pxc = (float)px + 0.5f;
-    const float pyc = (float)py + 0.5f;
-    bool done = !inside;
+    float pyc[PIX];
+#pragma unroll
+    for (int p = 0; p < PIX; ++p) pyc[p] = (float)(py0 + p) + 0.5f;
+    bool any = false;
+#pragma unroll
+    for (int p = 0; p < PIX; ++p) any |= live[p];
     for (int base = rg.x; base < rg.y; base += BATCH) {
-      if (__syncthreads_count(!done) == 0) break;
+      if (__syncthreads_count(any) == 0) break;
       const int n = min(BATCH, rg.y - base);
       for (int j = t; j < n; j += nthreads) {
         const uint32_t vi = vals[base + j];
         const float4 a = rec[3u * vi + 0u];
         const float4 b = rec[3u * vi + 1u];
         const float4 c = rec[3u * vi + 2u];
-        s0[j] = a;
-        s1[j] = b;
-        s2[j] = make_float4(c.x, c.y, power_limit(b.y), 0.0f);
+        s0[j] = a;                                              // x, y, i00, 2 i01
+        s1[j] = make_float4(b.x, b.y, power_limit(b.y), 0.0f);  // i11, opacity, power limit
+        s2[j] = make_float4(b.z, b.w, c.x, c.y);                // r, g, b, depth
         s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
                                         v.tile_w, rows, nwaves);
       }
@@ -486,50 +499,62 @@ __global__ __launch_bounds__(256) void k_composite(
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (!done) {
+      if (any) {
         for (uint32_t q = 0; q < cnt; ++q) {
           const int j = s_list[w][q];
-          const float4 a = s0[j];
-          const float4 b = s1[j];
+          const float4 a = s0[j];  // x, y, i00, 2 i01
+          const float4 b = s1[j];  // i11, opacity, power limit
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
-          const float dy = pyc - a.y;
-          float power = 0.5f * (a.z * dx * dx + a.w * dx * dy + b.x * dy * dy);
-          if (power < 0.0f) power = 0.0f;
-          const float4 c = s2[j];
-          if (power > c.z) continue;  // power > 4.5, or alpha < 1/255 (power_limit)
-          float alpha = b.y * splat_exp_neg(-power);
-          if (alpha > 0.99f) alpha = 0.99f;
-          // PixelAccumulator.blend, compositor.mojo:25-53
-          if (alpha < kMinAlpha) continue;
-          const float wgt = alpha * T;
-          C0 += b.z * wgt;
-          C1 += b.w * wgt;
-          C2 += c.x * wgt;
-          D += c.y * wgt;
-          T *= (1.0f - alpha);
-          if (T <= kTransmittanceEps) {
-            done = true;
-            ne = base + j - rg.x + 1;
-            break;
+          const float txx = a.z * dx * dx;
+          const float txy = a.w * dx;
+#pragma unroll
+          for (int p = 0; p < PIX; ++p) {
+            if (!live[p]) continue;
+            const float dy = pyc[p] - a.y;
+            float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
+            if (power < 0.0f) power = 0.0f;
+            if (power > b.z) continue;  // power > 4.5, or alpha < 1/255 (power_limit)
+            float alpha = b.y * splat_exp_neg(-power);
+            if (alpha > 0.99f) alpha = 0.99f;
+            // PixelAccumulator.blend, compositor.mojo:25-53
+            if (alpha < kMinAlpha) continue;
+            const float4 c = s2[j];  // r, g, b, depth
+            const float wgt = alpha * T[p];
+            C0[p] += c.x * wgt;
+            C1[p] += c.y * wgt;
+            C2[p] += c.z * wgt;
+            D[p] += c.w * wgt;
+            T[p] *= (1.0f - alpha);
+            if (T[p] <= kTransmittanceEps) {
+              live[p] = false;
+              ne[p] = base + j - rg.x + 1;
+            }
           }
+          any = false;
+#pragma unroll
+          for (int p = 0; p < PIX; ++p) any |= live[p];
+          if (!any) break;
         }
       }
     }
   }
-  if (!inside) return;
-  const size_t pix = v.pix_base + (size_t)py * v.width + px;
   // PixelAccumulator.finalize + RenderOutput writes, pipeline.mojo:182-229.
   // rasterize_scene returns its zero-filled output before any background fill
   // when the view has no tile instance (pipeline.mojo:63-65, 162-163).
   const bool white = v.white_background && spans[vw].instances > 0;
-  const float bg = white ? T : 0.0f;
-  rgb[3 * pix + 0] = C0 + bg;
-  rgb[3 * pix + 1] = C1 + bg;
-  rgb[3 * pix + 2] = C2 + bg;
-  if (depth) depth[pix] = v.compute_depth ? D : 0.0f;
-  if (t_final) t_final[pix] = T;
-  if (n_eval) n_eval[pix] = ne;
+#pragma unroll
+  for (int p = 0; p < PIX; ++p) {
+    if (px >= v.width || py0 + p >= v.height) continue;
+    const size_t pix = v.pix_base + (size_t)(py0 + p) * v.width + px;
+    const float bg = white ? T[p] : 0.0f;
+    rgb[3 * pix + 0] = C0[p] + bg;
+    rgb[3 * pix + 1] = C1[p] + bg;
+    rgb[3 * pix + 2] = C2[p] + bg;
+    if (depth) depth[pix] = v.compute_depth ? D[p] : 0.0f;
+    if (t_final) t_final[pix] = T[p];
+    if (n_eval) n_eval[pix] = ne[p];
+  }
 }
 
 __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
@@ -662,21 +687,22 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t thr
   const float4* rec = c->rec.as<float4>();
   const ViewParams* views = c->views.as<ViewParams>();
   const ViewSpan* spans = c->spans.as<ViewSpan>();
-  const dim3 grid((uint32_t)tiles), block(threads);
-  switch (batch) {
-    case 64:
-      hipLaunchKernelGGL(k_composite<64>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         (int)nviews, spans, rgb, depth, t_final, n_eval);
-      break;
-    case 128:
-      hipLaunchKernelGGL(k_composite<128>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         (int)nviews, spans, rgb, depth, t_final, n_eval);
-      break;
-    default:
-      hipLaunchKernelGGL(k_composite<256>, grid, block, 0, st, ranges, v0, v1, plan, rec, views,
-                         (int)nviews, spans, rgb, depth, t_final, n_eval);
-      break;
+  // two pixels per lane where the tile has at least 128 pixels
+  const bool two = threads % 128 == 0;
+  const dim3 grid((uint32_t)tiles), block(two ? threads / 2 : threads);
+#define LUMEN_COMPOSITE(B, P)                                                                  \
+  hipLaunchKernelGGL((k_composite<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec, views, \
+                     (int)nviews, spans, rgb, depth, t_final, n_eval)
+  if (two) {
+    if (batch == 64) LUMEN_COMPOSITE(64, 2);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 2);
+    else LUMEN_COMPOSITE(256, 2);
+  } else {
+    if (batch == 64) LUMEN_COMPOSITE(64, 1);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 1);
+    else LUMEN_COMPOSITE(256, 1);
   }
+#undef LUMEN_COMPOSITE
   LUMEN_LAUNCHED();
   return LUMEN_SUCCESS;
 }
@@ -784,7 +810,10 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
   float* tf = save ? c->t_final.as<float>() : nullptr;
   int32_t* ne = save ? c->n_eval.as<int32_t>() : nullptr;
-  const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : 256u;
+  // default LDS batch: 128 records (two pixels per lane keep 2 waves per
+  // block resident; 128 leaves room for 8 waves per SIMD), 256 for one pixel
+  const uint32_t tpix = cfg->tile_width * cfg->tile_height;
+  const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : (tpix % 128 == 0 ? 128u : 256u);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, false, st));
   LUMEN_TRY(composite_launch(batch, fs.tiles, cfg->tile_width * cfg->tile_height, st, c,
                              fs.nviews, out_rgb, out_depth, tf, ne));
@@ -996,7 +1025,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
                    &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
                    &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
                    &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis,
-                   &c->vmeta, &c->partial};
+                   &c->vmeta, &c->partial, &c->loss_part};
   for (DevBuf* b : all) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);

@@ -27,7 +27,7 @@ class RenderConfig:
     max_sh_degree: int = 3
     white_background: bool = False
     compute_depth: bool = True
-    batch_depth: int = 256  # compositor LDS batch (64, 128, 256)
+    batch_depth: int = 0  # compositor LDS batch (64, 128, 256); 0: the library's choice
 
     def ex_struct(self, flags: int = 0):
         c = _lib.LumenExRenderConfig()

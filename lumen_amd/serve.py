@@ -178,7 +178,7 @@ class RenderServer:
                 rgb, depth = rasterize_views(planes, sh, cams, self.config, R)
                 H, W = r.camera.config.height, r.camera.config.width
                 img = rgb.view(H, W, 3)
-                loss = combined_loss(img, r.target)
+                loss = combined_loss(img, r.target, rasterizer=R.rasterizer)
                 loss.backward()
                 r.rgb, r.depth = img.detach(), depth.detach().view(H, W)
                 r.grad_planes, r.grad_sh = planes.grad, sh.grad

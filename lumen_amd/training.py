@@ -6,8 +6,9 @@
   density control reads (src/optim/density_control.mojo:67).
 - RasterizeSplats is a torch.autograd.Function whose forward is lumen_ex_render
   and whose backward is lumen_ex_render_backward (both in liblumen.so).
-- l1_loss / ssim_window / combined_loss / psnr restate src/optim/loss.mojo
-  with torch ops (used on device, differentiable).
+- photometric_loss / l1_loss / ssim_window / combined_loss restate
+  src/optim/loss.mojo through lumen_ex_photometric_loss (values and
+  gradient in one HIP pass); psnr is a host-side metric.
 """
 from __future__ import annotations
 
@@ -122,44 +123,68 @@ def rasterize(planes: torch.Tensor, sh: torch.Tensor, camera: PinholeCamera,
 
 # ---- losses: src/optim/loss.mojo ------------------------------------------
 
-SSIM_C1 = 0.0001
-SSIM_C2 = 0.0009
+class PhotometricLoss(torch.autograd.Function):
+    """(pred, target) -> ((1 - lambda) L1 + lambda (1 - SSIM), L1, SSIM) through
+    lumen_ex_photometric_loss (one HIP pass computes the values and
+    d(loss)/d(pred)); only the first output is differentiable."""
+
+    @staticmethod
+    def forward(ctx, pred, target, lambda_dssim: float, rasterizer: Optional[Rasterizer]):
+        import ctypes
+        from . import _lib
+        if not pred.is_cuda or pred.dim() != 3 or pred.shape[2] != 3:
+            raise ValueError("photometric loss: pred must be a (H, W, 3) device tensor")
+        if target.shape != pred.shape:
+            raise ValueError("photometric loss: target shape differs from pred")
+        R = rasterizer or default_rasterizer()
+        p = pred.detach().float().contiguous()
+        t = target.detach().float().contiguous()
+        H, W = p.shape[0], p.shape[1]
+        out = torch.empty(3, dtype=torch.float32, device=p.device)
+        grad = torch.empty_like(p) if ctx.needs_input_grad[0] else None
+        st = _lib.lib().lumen_ex_photometric_loss(
+            R.handle, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(t.data_ptr()), W, H,
+            float(lambda_dssim), ctypes.c_void_p(out.data_ptr()),
+            ctypes.c_void_p(grad.data_ptr()) if grad is not None else None,
+            ctypes.c_void_p(torch.cuda.current_stream(p.device).cuda_stream))
+        _lib.check("lumen_ex_photometric_loss", st)
+        ctx.save_for_backward(grad if grad is not None else out)
+        ctx.has_grad = grad is not None
+        l1, ssim = out[0].clone(), out[1].clone()
+        ctx.mark_non_differentiable(l1, ssim)
+        return out[2].clone(), l1, ssim
+
+    @staticmethod
+    def backward(ctx, g_loss, g_l1, g_ssim):
+        if not ctx.has_grad:
+            return None, None, None, None
+        (grad,) = ctx.saved_tensors
+        return grad * g_loss, None, None, None
+
+
+def photometric_loss(pred: torch.Tensor, target: torch.Tensor, lambda_dssim: float = 0.2,
+                     rasterizer: Optional[Rasterizer] = None):
+    """((1 - lambda) L1 + lambda (1 - SSIM), L1, SSIM) of (H, W, 3) device images."""
+    return PhotometricLoss.apply(pred, target, float(lambda_dssim), rasterizer)
 
 
 def l1_loss(pred: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
     """Mean |pred - target| whose gradient is +1/N where pred >= target and
     -1/N elsewhere (compute_l1_loss, loss.mojo:11-35)."""
-    diff = pred - target
-    return torch.where(diff >= 0, diff, -diff).mean()
+    return photometric_loss(pred, target, 0.0)[0]
 
 
-def ssim_window(pred: torch.Tensor, target: torch.Tensor, win: int = 8) -> torch.Tensor:
-    """Mean SSIM over non-overlapping win x win windows with the three colour
+def ssim_window(pred: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
+    """Mean SSIM over non-overlapping 8x8 windows with the three colour
     channels pooled (compute_ssim_window, loss.mojo:65-133); partial windows
-    at the right/bottom edges are not counted.  pred/target: (H, W, 3)."""
-    H, W = pred.shape[0], pred.shape[1]
-    hy, wx = H // win, W // win
-    if hy == 0 or wx == 0:
-        return torch.ones((), device=pred.device, dtype=pred.dtype)
-
-    def blocks(x):
-        x = x[: hy * win, : wx * win]
-        return x.reshape(hy, win, wx, win, 3).permute(0, 2, 1, 3, 4).reshape(hy, wx, win * win * 3)
-
-    x, y = blocks(pred), blocks(target)
-    mx, my = x.mean(-1, keepdim=True), y.mean(-1, keepdim=True)
-    dx, dy = x - mx, y - my
-    vx, vy = (dx * dx).mean(-1), (dy * dy).mean(-1)
-    cxy = (dx * dy).mean(-1)
-    mx, my = mx[..., 0], my[..., 0]
-    num = (2.0 * mx * my + SSIM_C1) * (2.0 * cxy + SSIM_C2)
-    den = (mx * mx + my * my + SSIM_C1) * (vx + vy + SSIM_C2)
-    return (num / den).mean()
+    at the right/bottom edges are not counted."""
+    return 1.0 - photometric_loss(pred, target, 1.0)[0]
 
 
-def combined_loss(pred: torch.Tensor, target: torch.Tensor, lambda_dssim: float = 0.2) -> torch.Tensor:
+def combined_loss(pred: torch.Tensor, target: torch.Tensor, lambda_dssim: float = 0.2,
+                  rasterizer: Optional[Rasterizer] = None) -> torch.Tensor:
     """(1 - lambda) * L1 + lambda * (1 - SSIM) (compute_combined_loss, loss.mojo:136-144)."""
-    return (1.0 - lambda_dssim) * l1_loss(pred, target) + lambda_dssim * (1.0 - ssim_window(pred, target))
+    return photometric_loss(pred, target, lambda_dssim, rasterizer)[0]
 
 
 def psnr(pred: torch.Tensor, target: torch.Tensor) -> float:

@@ -148,6 +148,33 @@ def test_losses_match_reference_semantics(oracle):
     assert c == pytest.approx(oracle.lib().orc_combined_loss(ref_l1, ref_s, 0.2), rel=1e-5)
 
 
+@pytest.mark.parametrize("H,W", [(40, 56), (37, 61), (1080, 1920), (5, 7)])
+def test_photometric_loss_vs_float64_reference(H, W):
+    """lumen_ex_photometric_loss (values and gradient, one HIP pass) vs the
+    float64 PyTorch formulation of loss.mojo and its autograd gradient; sizes
+    with partial windows and with no full window."""
+    import torch
+    from loss_reference import combined_ref, l1_ref, ssim_ref
+    from lumen_amd.training import photometric_loss
+    g = torch.Generator().manual_seed(H * 1000 + W)
+    a = torch.rand((H, W, 3), generator=g)
+    b = (a + 0.1 * torch.randn((H, W, 3), generator=g)).clamp(0, 1)
+    b[0, 0, 0] = a[0, 0, 0]  # diff == 0 takes +1/N
+    pa = a.cuda().requires_grad_(True)
+    loss, l1, ssim = photometric_loss(pa, b.cuda(), 0.2)
+    loss.backward()
+    ra = a.double().requires_grad_(True)
+    ref = combined_ref(ra, b.double())
+    ref.backward()
+    assert float(l1) == pytest.approx(float(l1_ref(a.double(), b.double())), rel=2e-6)
+    assert float(ssim) == pytest.approx(float(ssim_ref(a.double(), b.double())), rel=2e-5, abs=1e-6)
+    assert float(loss) == pytest.approx(float(ref), rel=2e-5)
+    got = pa.grad.cpu().double()
+    err = float((got - ra.grad).norm() / ra.grad.norm())
+    say(f"[loss] {H}x{W} rel grad err {err:.2e}")
+    assert err < 1e-4
+
+
 def test_fwd_bwd_1m_1080p_descends(rasterizer):
     """The 1M-splat 1080p training step: L1 + D-SSIM against a render of a
     perturbed copy; gradients finite, and a small step on the means lowers the

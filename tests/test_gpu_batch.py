@@ -157,3 +157,34 @@ def test_autotune_configs_agree(renderer):
         renderer.synchronize()
         imgs.append(out.rgb.clone())
     assert torch.equal(imgs[0], imgs[1]) and torch.equal(imgs[0], imgs[2])
+
+
+def test_batch_autograd_releases_memory(renderer):
+    """Repeated forward+backward through rasterize_views keeps the live device
+    memory flat (no reference cycle through the autograd context)."""
+    import gc
+    import torch
+    from lumen_amd.batch import rasterize_views
+    scene = random_scene(20_000, 1.0, 96)
+    cams = mixed_cameras()[:2]
+    planes0 = torch.from_numpy(scene.planes).cuda()
+    sh0 = torch.from_numpy(scene.sh).cuda()
+
+    def step(graph):
+        planes = planes0.clone().requires_grad_(True)
+        sh = sh0.clone().requires_grad_(True)
+        rgb, depth = rasterize_views(planes, sh, cams, RenderConfig(), renderer, graph=graph)
+        (rgb.sum() + depth.sum()).backward()
+
+    for graph in (False, True):
+        step(graph)
+        torch.cuda.synchronize()
+        gc.collect()
+        base = torch.cuda.memory_allocated()
+        for _ in range(6):
+            step(graph)
+        torch.cuda.synchronize()
+        gc.collect()
+        grown = torch.cuda.memory_allocated() - base
+        say(f"[batch] autograd memory growth over 6 steps: {grown} B")
+        assert grown < 1 << 20
