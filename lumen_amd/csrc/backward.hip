@@ -139,10 +139,13 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const float4 a = rec[3u * vi + 0u];
       const float4 b = rec[3u * vi + 1u];
       const float4 c = rec[3u * vi + 2u];
-      s0[j] = a;                                              // x, y, i00, 2 i01
-      s1[j] = make_float4(b.x, b.y, power_limit(b.y), 0.0f);  // i11, opacity, power limit
-      s2[j] = make_float4(b.z, b.w, c.x, c.y);                // r, g, b, depth
-      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
+      const float lim = power_limit(b.y);
+      float hx = c.z, hy = c.w;
+      limit_box(lim, hx, hy);
+      s0[j] = a;                                // x, y, i00, 2 i01
+      s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
+      s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
+      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, tx * v.tile_w, ty * v.tile_h,
                                       v.tile_w, rows, nwaves);
     }
     __syncthreads();

@@ -190,14 +190,17 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
   const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
   if (threadIdx.x == 0) {
-    int32_t dg[8];
-    int np = 0;
-    if (V > 0) {
-      np = varying_digits(bounds[0], bounds[1], 0, 4, dg);
-      np += varying_digits(bounds[2], bounds[3], 4, 4, dg + np);
+    SortPlan p;
+    plan_digits(V > 0, bounds[0], bounds[1], bounds[2], bounds[3], p);
+    plan->passes = p.passes;
+    for (int k = 0; k < 8; ++k) {
+      plan->shift[k] = p.shift[k];
+      plan->width[k] = p.width[k];
     }
-    plan->passes = np;
-    for (int k = 0; k < 8; ++k) plan->digit[k] = k < np ? dg[k] : 0;
+    plan->mode = p.mode;
+    plan->tmin = p.tmin;
+    plan->dmin = p.dmin;
+    plan->dbits = p.dbits;
     plan->m = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
     if ((uint64_t)M > m_cap || (uint64_t)V > v_cap) plan->overflow = 1u;
@@ -276,6 +279,11 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
     pz = s.pz[i];
     op = s.op[i];
   }
+  // mode 1 plans sort 32-bit frame keys (tile - tmin) << dbits | (depth key - dmin)
+  const bool frame_keys = plan->mode == 1;
+  const uint32_t kt0 = plan->tmin, kd0 = plan->dmin;
+  const int kdb = plan->dbits;
+  uint32_t* __restrict__ keys32 = reinterpret_cast<uint32_t*>(keys);
   bool have_sh = false;
   float c[48];
   for (int vw = 0; vw < nviews; ++vw) {
@@ -369,14 +377,25 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       const uint32_t row = local / wtiles;
       const uint32_t ty = min_ty + row;
       const uint32_t tx = min_tx + (local - row * wtiles);
-      keys[base + p] = ((uint64_t)(v.tile_base + ty * (uint32_t)v.tiles_x + tx) << 32) |
-                       (uint64_t)s_dk[w][lo];
+      const uint32_t tile = v.tile_base + ty * (uint32_t)v.tiles_x + tx;
+      if (frame_keys)
+        keys32[base + p] = ((tile - kt0) << kdb) | (s_dk[w][lo] - kd0);
+      else
+        keys[base + p] = ((uint64_t)tile << 32) | (uint64_t)s_dk[w][lo];
       vals[base + p] = s_vi[w][lo];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+}
+
+// Tile of sorted key i: the high half of a 64-bit key (mode 0) or the high
+// bits of a 32-bit frame key plus tmin (mode 1).
+__device__ __forceinline__ uint32_t sorted_tile(const void* keys, uint32_t i, bool frame_keys,
+                                                uint32_t tmin, int dbits) {
+  if (frame_keys) return (static_cast<const uint32_t*>(keys)[i] >> dbits) + tmin;
+  return (uint32_t)(static_cast<const uint64_t*>(keys)[i] >> 32);
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
@@ -386,11 +405,21 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const uint32_t m = plan->m;
   if (i >= m || plan->overflow) return;
-  const uint64_t* __restrict__ keys = (plan->passes & 1) ? k1 : k0;
-  const uint32_t tile = (uint32_t)(keys[i] >> 32);
+  const void* keys = (plan->passes & 1) ? (const void*)k1 : (const void*)k0;
+  const bool fk = plan->mode == 1;
+  const uint32_t tmin = plan->tmin;
+  const int db = plan->dbits;
+  const uint32_t tile = sorted_tile(keys, i, fk, tmin, db);
   if (tile >= tiles) return;
-  if (i == 0 || (uint32_t)(keys[i - 1] >> 32) != tile) ranges[tile].x = (int32_t)i;
-  if (i == m - 1 || (uint32_t)(keys[i + 1] >> 32) != tile) ranges[tile].y = (int32_t)(i + 1);
+  if (i == 0 || sorted_tile(keys, i - 1, fk, tmin, db) != tile) ranges[tile].x = (int32_t)i;
+  if (i == m - 1 || sorted_tile(keys, i + 1, fk, tmin, db) != tile) ranges[tile].y = (int32_t)(i + 1);
+}
+
+// 32-bit frame keys back to the 64-bit keys (for lumen_ex_copy_stage).
+__global__ void k_decode_keys(const uint32_t* __restrict__ k32, uint32_t n,
+                              const SortPlan* __restrict__ plan, uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) out[i] = frame_key_to_key(k32[i], *plan);
 }
 
 // Ranges of an arbitrary sorted key array (lumen_ex_build_ranges).
@@ -480,10 +509,13 @@ __global__ __launch_bounds__(256) void k_composite(
         const float4 a = rec[3u * vi + 0u];
         const float4 b = rec[3u * vi + 1u];
         const float4 c = rec[3u * vi + 2u];
-        s0[j] = a;                                              // x, y, i00, 2 i01
-        s1[j] = make_float4(b.x, b.y, power_limit(b.y), 0.0f);  // i11, opacity, power limit
-        s2[j] = make_float4(b.z, b.w, c.x, c.y);                // r, g, b, depth
-        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, c.z, c.w, tx * v.tile_w, ty * v.tile_h,
+        const float lim = power_limit(b.y);
+        float hx = c.z, hy = c.w;
+        limit_box(lim, hx, hy);
+        s0[j] = a;                                // x, y, i00, 2 i01
+        s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
+        s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
+        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, tx * v.tile_w, ty * v.tile_h,
                                         v.tile_w, rows, nwaves);
       }
       __syncthreads();
@@ -581,7 +613,11 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
 
 __global__ void k_set_plan(SortPlan* plan, int passes, const int32_t* digits, uint32_t m) {
   plan->passes = passes;
-  for (int k = 0; k < 8; ++k) plan->digit[k] = k < passes ? digits[k] : 0;
+  for (int k = 0; k < 8; ++k) {
+    plan->shift[k] = k < passes ? 8 * digits[k] : 0;
+    plan->width[k] = k < passes ? 8 : 0;
+  }
+  plan->mode = 0;
   plan->m = m;
   plan->v = 0;
   plan->overflow = 0;
@@ -765,8 +801,8 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
 // given capacities (exact after a host read-back, reserved in a graph).
 static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
                                 const LumenExRenderConfig* cfg, uint64_t v_cap, uint64_t m_cap,
-                                int sort_slots, bool keep, float* out_rgb, float* out_depth,
-                                hipStream_t st) {
+                                int sort_slots, int sort_modes, bool keep, float* out_rgb,
+                                float* out_depth, hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, c->packed.as<uint64_t>(),
@@ -798,7 +834,7 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
     uint64_t* kb[2] = {c->keys[0].as<uint64_t>(), c->keys[1].as<uint64_t>()};
     uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
-    LUMEN_TRY(radix_sort_planned(kb, vb, plan, m_cap, sort_slots,
+    LUMEN_TRY(radix_sort_planned(kb, vb, plan, m_cap, sort_slots, sort_modes,
                                  SortScratch{&c->hist, &c->hist_part}, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
@@ -863,14 +899,14 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
                   (unsigned long long)M);
   }
   LUMEN_TRY(ensure_capacity(c, V, M));
-  int32_t dg[8];
-  int np = 0;
-  if (V > 0) {
-    np = varying_digits(hb[0], hb[1], 0, 4, dg);
-    np += varying_digits(hb[2], hb[3], 4, 4, dg + np);
-  }
+  // the plan k_plan will write, known here from the bounds: only its passes
+  // and key width are enqueued
+  SortPlan hp;
+  plan_digits(V > 0, hb[0], hb[1], hb[2], hb[3], hp);
+  const int np = hp.passes;
   const bool keep = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0 && M > 0;
-  LUMEN_TRY(enqueue_rest(c, sd, fs, cfg, V, M, np, keep, out_rgb, out_depth, st));
+  LUMEN_TRY(enqueue_rest(c, sd, fs, cfg, V, M, np, hp.mode == 1 ? 2 : 1, keep, out_rgb,
+                         out_depth, st));
   c->V = V;
   c->M = M;
   c->T = fs.tiles;
@@ -946,7 +982,7 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   // sort scratch for the capacity, allocated before capture
   {
     const uint32_t nb = div_up(m_cap, kSortTile);
-    const uint64_t hlen = (uint64_t)nb * 256u;
+    const uint64_t hlen = (uint64_t)nb * kDigitBins;
     LUMEN_TRY(ensure(c->hist, hlen * sizeof(uint32_t)));
     LUMEN_TRY(ensure(c->hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
   }
@@ -958,7 +994,7 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   LUMEN_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   LumenStatus s1 = enqueue_count(c, sd, fs, st);
   LumenStatus s2 = s1 == LUMEN_SUCCESS
-                       ? enqueue_rest(c, sd, fs, &pc, v_cap, m_cap, slots, false, out_rgb,
+                       ? enqueue_rest(c, sd, fs, &pc, v_cap, m_cap, slots, 3, false, out_rgb,
                                       out_depth, st)
                        : s1;
   hipGraph_t g = nullptr;
@@ -1144,7 +1180,7 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   LUMEN_LAUNCHED();
   uint64_t* kb[2] = {keys, c->keys[1].as<uint64_t>()};
   uint32_t* vb[2] = {values, c->vals[1].as<uint32_t>()};
-  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), n, nd,
+  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), n, nd, 1,
                                SortScratch{&c->hist, &c->hist_part}, st));
   if (nd & 1) {
     LUMEN_HIP(hipMemcpyAsync(keys, kb[1], n * 8, hipMemcpyDeviceToDevice, st));
@@ -1179,6 +1215,29 @@ LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint3
   return LUMEN_SUCCESS;
 }
 
+// For a frame sorted as 32-bit frame keys (mode 1), decodes the c->M keys at
+// k32 into 64-bit keys in buffer keys[into] (the other ping-pong buffer for
+// the sorted keys; for the kept unsorted keys, into = -1 takes whichever
+// buffer does not hold the sorted result) and points *src there.  Mode 0
+// leaves *src as it is.
+static LumenStatus decode_frame_keys(LumenExContext* c, const void* k32, int into,
+                                     const void** src) {
+  SortPlan p;
+  LUMEN_HIP(hipSetDevice(c->device));
+  LUMEN_HIP(hipDeviceSynchronize());
+  LUMEN_HIP(hipMemcpy(&p, c->plan.ptr, sizeof(p), hipMemcpyDeviceToHost));
+  if (p.mode != 1) return LUMEN_SUCCESS;
+  if (into < 0) into = 1 - (p.passes & 1);
+  uint64_t* out = c->keys[into].as<uint64_t>();
+  hipLaunchKernelGGL(k_decode_keys, dim3(div_up(c->M, 256)), dim3(256), 0, nullptr,
+                     static_cast<const uint32_t*>(k32), (uint32_t)c->M, c->plan.as<SortPlan>(),
+                     out);
+  LUMEN_LAUNCHED();
+  LUMEN_HIP(hipDeviceSynchronize());
+  *src = out;
+  return LUMEN_SUCCESS;
+}
+
 LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t bytes) {
   clear_error();
   if (!c || !dst) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_copy_stage: null argument");
@@ -1194,7 +1253,9 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t 
     case LUMEN_EX_STAGE_RADIUS: src = c->radius.ptr; want = c->V * 4; break;
     case LUMEN_EX_STAGE_KEYS_UNSORTED:
       if (!c->kept && c->M) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_KEEP_STAGES)");
-      src = c->keys_keep.ptr; want = c->M * 8; break;
+      src = c->keys_keep.ptr; want = c->M * 8;
+      if (c->M) LUMEN_TRY(decode_frame_keys(c, c->keys_keep.ptr, -1, &src));
+      break;
     case LUMEN_EX_STAGE_VALS_UNSORTED:
       if (!c->kept && c->M) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_KEEP_STAGES)");
       src = c->vals_keep.ptr; want = c->M * 4; break;
@@ -1210,8 +1271,14 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t 
         p.passes = fb;
       }
       const int b = p.passes & 1;
-      if (stage == LUMEN_EX_STAGE_KEYS) { src = c->keys[b].ptr; want = c->M * 8; }
-      else { src = c->vals[b].ptr; want = c->M * 4; }
+      if (stage == LUMEN_EX_STAGE_KEYS) {
+        src = c->keys[b].ptr;
+        want = c->M * 8;
+        if (c->M) LUMEN_TRY(decode_frame_keys(c, c->keys[b].ptr, 1 - b, &src));
+      } else {
+        src = c->vals[b].ptr;
+        want = c->M * 4;
+      }
       break;
     }
     case LUMEN_EX_STAGE_RANGES: src = c->ranges.ptr; want = c->T * 8; break;

@@ -7,16 +7,29 @@
 
 namespace lumen {
 
+constexpr int kMaxDigitBits = 9;                 // widest digit a pass sorts
+constexpr int kDigitBins = 1 << kMaxDigitBits;   // histogram rows per block
+
 // Written on the device by the frame's planning kernel (or by the host path
 // of lumen_ex_radix_sort), read by every later kernel of the frame, so that a
 // frame needs no host round trip once its buffers are sized.
+//
+// mode 0 sorts the 64-bit keys tile << 32 | depth_key in 8-bit digits.
+// mode 1 sorts 32-bit frame keys (tile - tmin) << dbits | (depth_key - dmin),
+// used when the tile ids and depth keys of the frame vary in at most 32 bits
+// together: both transforms are monotone and the fields do not overlap, so
+// equal 64-bit keys map to equal frame keys and the stable order is the same.
 struct SortPlan {
   int32_t passes;      // digit passes to run
-  int32_t digit[8];    // digit index (0 = bits 0..7) of each pass
+  int32_t shift[8];    // first key bit of each pass's digit
+  int32_t width[8];    // digit width in bits (<= kMaxDigitBits)
   uint32_t m;          // keys to sort (tile instances of the frame)
   uint32_t v;          // visible splats of the frame
   uint32_t overflow;   // set when the frame exceeded a reserved capacity
-  uint32_t pad[4];
+  int32_t mode;        // 0: 64-bit keys, 1: 32-bit frame keys
+  uint32_t tmin, dmin; // frame-key offsets (mode 1)
+  int32_t dbits;       // depth bits of a frame key (mode 1)
+  uint32_t pad;
 };
 
 // Digits that can differ between keys lying in [lo, hi]: those at or below the
@@ -31,6 +44,60 @@ __host__ __device__ inline int varying_digits(uint64_t lo, uint64_t hi, int digi
   int n = 0;
   for (int d = 0; d <= top / 8 && d < max_digits; ++d) out[n++] = digit_base + d;
   return n;
+}
+
+__host__ __device__ inline int bit_width(uint32_t x) {
+  int n = 0;
+  while (x) {
+    ++n;
+    x >>= 1;
+  }
+  return n;
+}
+
+// The sort of a frame whose depth keys lie in [dmin, dmax] and tile ids in
+// [tmin, tmax] (nothing to sort when there is no key): mode 1 with
+// ceil(bits / 9) passes of equal width when the two ranges need at most 32
+// bits, else mode 0 over the varying 8-bit digits of the 64-bit key.  Fills
+// passes, shift, width, mode, tmin, dmin, dbits.
+__host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dmax, uint32_t tmin,
+                                            uint32_t tmax, SortPlan& p) {
+  for (int k = 0; k < 8; ++k) p.shift[k] = p.width[k] = 0;
+  p.passes = 0;
+  p.mode = 1;
+  p.tmin = tmin;
+  p.dmin = dmin;
+  p.dbits = 0;
+  if (!any) return;
+  const int db = bit_width(dmax - dmin), tb = bit_width(tmax - tmin);
+  if (db + tb <= 32) {
+    const int total = db + tb;
+    const int np = (total + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int w = np ? (total + np - 1) / np : 0;
+    p.dbits = db;
+    p.passes = np;
+    for (int k = 0; k < np; ++k) {
+      p.shift[k] = k * w;
+      p.width[k] = total - k * w < w ? total - k * w : w;
+    }
+    return;
+  }
+  p.mode = 0;
+  int32_t dg[8];
+  int np = varying_digits(dmin, dmax, 0, 4, dg);
+  np += varying_digits(tmin, tmax, 4, 4, dg + np);
+  p.passes = np;
+  for (int k = 0; k < np; ++k) {
+    p.shift[k] = 8 * dg[k];
+    p.width[k] = 8;
+  }
+}
+
+// A frame key (mode 1) back to the 64-bit key.
+__host__ __device__ inline uint64_t frame_key_to_key(uint32_t k, const SortPlan& p) {
+  const uint32_t dmask = p.dbits ? (0xFFFFFFFFu >> (32 - p.dbits)) : 0u;
+  const uint32_t tile = (p.dbits < 32 ? (k >> p.dbits) : 0u) + p.tmin;
+  return ((uint64_t)tile << 32) | (uint64_t)((k & dmask) + p.dmin);
 }
 
 }  // namespace lumen

@@ -1,6 +1,7 @@
-// sort.hpp -- stable LSD radix sort of (uint64 key, uint32 value) pairs with
-// 8-bit digits, the device form of radix_sort_64bit
-// (src/raster/radix_sort.mojo:39-87).
+// sort.hpp -- stable LSD radix sort of (key, uint32 value) pairs, the device
+// form of radix_sort_64bit (src/raster/radix_sort.mojo:39-87): 64-bit keys
+// in 8-bit digits, or a frame's 32-bit keys in digits of up to 9 bits (see
+// SortPlan).
 //
 // Only digits that can differ between keys are sorted: a stable pass over a
 // digit that is equal in all keys is the identity.  The digit plan, the
@@ -8,13 +9,13 @@
 // (SortPlan), so a frame's whole sort can be captured in a hipGraph with no
 // host round trip; grids are sized for a capacity and surplus blocks exit.
 // Each pass:
-//   k_sort_hist     per-block 256-bin histogram of the digit (LDS atomics on
+//   k_sort_hist     per-block histogram of the digit (LDS atomics on
 //                   per-wave sub-histograms), stored digit-major so that one
 //                   exclusive scan gives every block's start offset per digit
-//   exclusive_scan  over the 256 x blocks table
+//   exclusive_scan  over the kDigitBins x blocks table
 //   k_sort_scatter  stable rank inside the block: each wave owns 1024
-//                   consecutive keys and ranks them 64 at a time with 8
-//                   ballots (lanes with equal digits are peers; rank = peers
+//                   consecutive keys and ranks them 64 at a time with one
+//                   ballot per digit bit (lanes with equal digits are peers; rank = peers
 //                   below this lane) and wave-private digit counters in LDS;
 //                   one block scan then orders the waves, the block's keys are
 //                   staged in LDS in digit order and written out so that each
@@ -36,78 +37,92 @@ __device__ __forceinline__ uint32_t sort_blocks(uint32_t m) {
   return (m + kSortTile - 1) / kSortTile;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(uint64_t* k0, uint64_t* k1,
+// K = uint64_t sorts mode-0 plans (64-bit keys), K = uint32_t mode-1 plans
+// (32-bit frame keys held in the same buffers); a kernel of the other mode
+// returns at once, so a graph can hold both.
+template <typename K>
+__device__ __forceinline__ bool sort_mode_is(const SortPlan* plan) {
+  return plan->mode == (sizeof(K) == 4 ? 1 : 0);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K* k1,
                                                            const SortPlan* __restrict__ plan,
                                                            int pass, uint32_t* __restrict__ hist,
                                                            uint32_t nb_cap) {
-  __shared__ uint32_t h[4][256];
-  if (pass >= plan->passes) return;
+  __shared__ uint32_t h[4][kDigitBins];
+  if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
   const uint32_t m = plan->m;
   const int t = threadIdx.x;
   if (blockIdx.x >= sort_blocks(m)) {  // surplus block of a capacity-sized grid
-    hist[(uint32_t)t * nb_cap + blockIdx.x] = 0u;
+    for (int d = t; d < kDigitBins; d += kSortThreads) hist[(uint32_t)d * nb_cap + blockIdx.x] = 0u;
     return;
   }
-  const uint64_t* __restrict__ keys = (pass & 1) ? k1 : k0;
-  const int shift = 8 * plan->digit[pass];
+  const K* __restrict__ keys = (pass & 1) ? k1 : k0;
+  const int shift = plan->shift[pass];
+  const uint32_t mask = (1u << plan->width[pass]) - 1u;
   const int w = t >> 6;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) h[i][t] = 0;
+  for (int d = t; d < 4 * kDigitBins; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
   const uint32_t base = blockIdx.x * kSortTile;
 #pragma unroll 4
   for (int j = 0; j < kSortItems; ++j) {
     const uint32_t idx = base + j * kSortThreads + t;
-    if (idx < m) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & 0xFFu], 1u);
+    if (idx < m) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & mask], 1u);
   }
   __syncthreads();
-  hist[(uint32_t)t * nb_cap + blockIdx.x] = h[0][t] + h[1][t] + h[2][t] + h[3][t];
+  for (int d = t; d < kDigitBins; d += kSortThreads)
+    hist[(uint32_t)d * nb_cap + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
+template <typename K>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
-    uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
-    int pass, const uint32_t* __restrict__ offsets, uint32_t nb_cap) {
-  __shared__ uint64_t s_key[kSortTile];
+    K* k0, K* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan, int pass,
+    const uint32_t* __restrict__ offsets, uint32_t nb_cap) {
+  __shared__ K s_key[kSortTile];
   __shared__ uint32_t s_val[kSortTile];
-  __shared__ uint32_t s_cnt[4][256];  // per-wave digit counts, then per-wave starts
-  __shared__ uint32_t s_start[256];   // block-local start of each digit
-  __shared__ uint32_t s_glob[256];    // global start of each digit for this block
+  __shared__ uint32_t s_cnt[4][kDigitBins];  // per-wave digit counts, then per-wave starts
+  __shared__ uint32_t s_start[kDigitBins];   // block-local start of each digit
+  __shared__ uint32_t s_glob[kDigitBins];    // global start of each digit for this block
   __shared__ uint32_t s_w[4];
-  if (pass >= plan->passes) return;
+  if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
   const uint32_t m = plan->m;
   if (blockIdx.x >= sort_blocks(m)) return;
-  const uint64_t* __restrict__ kin = (pass & 1) ? k1 : k0;
+  const K* __restrict__ kin = (pass & 1) ? k1 : k0;
   const uint32_t* __restrict__ vin = (pass & 1) ? v1 : v0;
-  uint64_t* __restrict__ kout = (pass & 1) ? k0 : k1;
+  K* __restrict__ kout = (pass & 1) ? k0 : k1;
   uint32_t* __restrict__ vout = (pass & 1) ? v0 : v1;
-  const int shift = 8 * plan->digit[pass];
+  const int shift = plan->shift[pass];
+  const int width = plan->width[pass];
+  const uint32_t mask = (1u << width) - 1u;
   const int t = threadIdx.x;
   const int w = t >> 6;
   const int lane = t & 63;
   const uint64_t lt = lanemask_lt();
-  s_glob[t] = offsets[(uint32_t)t * nb_cap + blockIdx.x];
+  for (int d = t; d < kDigitBins; d += kSortThreads) {
+    s_glob[d] = offsets[(uint32_t)d * nb_cap + blockIdx.x];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) s_cnt[i][t] = 0u;
+    for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0u;
+  }
   __syncthreads();
   // wave w owns keys [base + 1024 w, base + 1024 (w + 1)), 64 per round
   const uint32_t base = blockIdx.x * kSortTile;
   const uint32_t wbase = base + (uint32_t)w * (kSortTile / 4);
-  uint64_t kk[kSortItems];
+  K kk[kSortItems];
   uint32_t vv[kSortItems];
   uint32_t off[kSortItems];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const uint32_t idx = wbase + (uint32_t)j * 64u + (uint32_t)lane;
-    kk[j] = idx < m ? kin[idx] : 0ull;
+    kk[j] = idx < m ? kin[idx] : K(0);
     vv[j] = idx < m ? vin[idx] : 0u;
   }
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
-    const uint32_t d = (uint32_t)(kk[j] >> shift) & 0xFFu;
+    const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < width; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
@@ -124,21 +139,34 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
-  {  // thread t: digit t's block-local start and each wave's start within it
-    const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+  {  // thread t: digits 2t and 2t + 1, their block-local starts and each wave's start
+    constexpr int kPer = kDigitBins / kSortThreads;
+    uint32_t c[kPer][4], sum = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        c[e][i] = s_cnt[i][kPer * t + e];
+        sum += c[e][i];
+      }
     uint32_t all;
-    const uint32_t st = block_exclusive_scan<uint32_t, kSortThreads>(c0 + c1 + c2 + c3, s_w, all);
-    s_start[t] = st;
-    s_cnt[0][t] = st;
-    s_cnt[1][t] = st + c0;
-    s_cnt[2][t] = st + c0 + c1;
-    s_cnt[3][t] = st + c0 + c1 + c2;
+    uint32_t st = block_exclusive_scan<uint32_t, kSortThreads>(sum, s_w, all);
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int d = kPer * t + e;
+      s_start[d] = st;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        s_cnt[i][d] = st;
+        st += c[e][i];
+      }
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     if (wbase + (uint32_t)j * 64u + (uint32_t)lane < m) {
-      const uint32_t d = (uint32_t)(kk[j] >> shift) & 0xFFu;
+      const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
       const uint32_t pos = s_cnt[w][d] + off[j];
       s_key[pos] = kk[j];
       s_val[pos] = vv[j];
@@ -149,8 +177,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   // addresses of the output
   const uint32_t nblk = min((uint32_t)kSortTile, m - base);
   for (uint32_t i = t; i < nblk; i += kSortThreads) {
-    const uint64_t key = s_key[i];
-    const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+    const K key = s_key[i];
+    const uint32_t d = (uint32_t)(key >> shift) & mask;
     const uint32_t dst = s_glob[d] + (i - s_start[d]);
     kout[dst] = key;
     vout[dst] = s_val[i];
@@ -163,26 +191,44 @@ struct SortScratch {
 };
 
 // Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
-// memory; the result ends in buffer (plan->passes & 1).  m_cap bounds the
-// number of keys the plan may name.
+// memory; the result ends in buffer (plan->passes & 1), as 64-bit keys
+// (mode 0) or 32-bit frame keys (mode 1) in the same buffers.  `modes` says
+// which key widths to enqueue (bit 0: 64-bit, bit 1: 32-bit; a graph, which
+// cannot know the frame's mode, takes both).  m_cap bounds the number of
+// keys the plan may name.
 inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
-                                      uint64_t m_cap, int slots, SortScratch sc,
+                                      uint64_t m_cap, int slots, int modes, SortScratch sc,
                                       hipStream_t st) {
   if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
   const uint32_t nb = div_up(m_cap, kSortTile);
-  const uint64_t hlen = (uint64_t)nb * 256u;
+  const uint64_t hlen = (uint64_t)nb * kDigitBins;
   LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
   LUMEN_TRY(ensure(*sc.hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
   uint32_t* hist = sc.hist->as<uint32_t>();
   uint32_t* part = sc.hist_part->as<uint32_t>();
+  uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};
   for (int p = 0; p < slots; ++p) {
-    hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1], plan, p,
-                       hist, nb);
-    LUMEN_LAUNCHED();
+    if (modes & 1) {
+      hipLaunchKernelGGL(k_sort_hist<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1],
+                         plan, p, hist, nb);
+      LUMEN_LAUNCHED();
+    }
+    if (modes & 2) {
+      hipLaunchKernelGGL(k_sort_hist<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
+                         k32[1], plan, p, hist, nb);
+      LUMEN_LAUNCHED();
+    }
     LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st));
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1], v[0],
-                       v[1], plan, p, hist, nb);
-    LUMEN_LAUNCHED();
+    if (modes & 1) {
+      hipLaunchKernelGGL(k_sort_scatter<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0],
+                         k[1], v[0], v[1], plan, p, hist, nb);
+      LUMEN_LAUNCHED();
+    }
+    if (modes & 2) {
+      hipLaunchKernelGGL(k_sort_scatter<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
+                         k32[1], v[0], v[1], plan, p, hist, nb);
+      LUMEN_LAUNCHED();
+    }
   }
   return LUMEN_SUCCESS;
 }

@@ -340,6 +340,17 @@ __device__ __forceinline__ float power_limit(float op) {
   return l < 4.5f ? l : 4.5f;
 }
 
+// The splat's box for a power limit below 4.5: the record's half-widths are
+// the 3-sigma (power 4.5) extents times 1.02 plus 0.02 px; the extents of
+// power <= lim are sqrt(lim / 4.5) times the 3-sigma ones, with the same
+// margins.  Infinite (unbounded) boxes stay infinite.
+__device__ __forceinline__ void limit_box(float lim, float& hx, float& hy) {
+  if (!(lim < 4.5f)) return;
+  const float s = sqrtf(lim * (1.0f / 4.5f));
+  hx = (hx - 0.02f) * s + 0.02f;
+  hy = (hy - 0.02f) * s + 0.02f;
+}
+
 // Bit w of the result: the splat's box (centre +- (hx, hy), outside of which
 // its power exceeds 4.5 at every pixel centre) meets the pixel centres of
 // wave w's strip of the tile (64 / tile_w full rows).  A wave whose bit is

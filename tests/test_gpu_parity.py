@@ -235,3 +235,20 @@ def test_build_ranges_kat(rasterizer, oracle):
                      [(0, 1), (0, 2), (2, 3), (2, 4), (2, 5), (3, 6), (3, 7)]], np.uint64)
     r = build_ranges(torch.from_numpy(keys.view(np.int64)).cuda(), 4).cpu().numpy()
     assert r.tolist() == [[0, 2], [-1, -1], [2, 5], [5, 7]]
+
+
+def test_wide_keys_take_the_64bit_sort(rasterizer, oracle):
+    """Tile ids and depth keys that need more than 32 bits together: the
+    frame is sorted as 64-bit keys in 8-bit digits (the other frames above as
+    32-bit frame keys); both give the oracle's order."""
+    sc = random_scene(30_000, 1000.0, 81)
+    sc.planes[3:6] *= 1000.0
+    cam = look_at(CameraConfig(640, 480, 600.0, 600.0, 320.0, 240.0, 0.01, 1.0e6),
+                  (0.0, 0.0, 5000.0))
+    ref, _ = compare(rasterizer, oracle, sc, cam)
+    keys = ref["keys"]
+    tiles = (keys >> np.uint64(32)).astype(np.int64)
+    depth = (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    bits = int(tiles.max() - tiles.min()).bit_length() + int(depth.max() - depth.min()).bit_length()
+    say(f"[parity] wide keys: {keys.size} keys, {bits} varying bits")
+    assert bits > 32

@@ -498,3 +498,37 @@ def test_power_limit_is_conservative(oracle):
         assert np.all(alpha < np.float32(0.00392156862)), step
         checked += int(keep.sum())
     assert checked > 100_000
+
+
+def test_limit_box_is_conservative():
+    """The compositors skip a splat for a wave whose pixel centres all lie
+    outside its box; with the power limit the box half-widths are
+    (h - 0.02) sqrt(lim / 4.5) + 0.02 of the 3-sigma box h = 3 sqrt(i11 / det)
+    1.02 + 0.02.  Just outside that box the power, computed in float32 as the
+    compositor does, exceeds the limit even at the row that minimises it."""
+    rng = np.random.default_rng(78)
+    f32 = np.float32
+    n = 200_000
+    # inverse covariances with condition numbers up to ~1e4 (the box's validity range)
+    lam1 = np.exp(rng.uniform(np.log(1e-4), np.log(10.0), n))
+    lam2 = lam1 * np.exp(rng.uniform(0.0, np.log(9e3), n))
+    th = rng.uniform(0, np.pi, n)
+    c, s = np.cos(th), np.sin(th)
+    i00 = (lam1 * c * c + lam2 * s * s).astype(f32)
+    i11 = (lam1 * s * s + lam2 * c * c).astype(f32)
+    i01 = ((lam1 - lam2) * c * s).astype(f32)
+    det = (i00 * i11 - i01 * i01).astype(f32)
+    tr = (i00 + i11).astype(f32)
+    ok = (i00 > 0) & (i11 > 0) & (det > 0) & (tr * tr <= f32(1e4) * det)
+    i00, i11, i01, det = i00[ok], i11[ok], i01[ok], det[ok]
+    op = rng.uniform(1.0 / 255.0, 1.0, i00.size).astype(f32)
+    lim = np.minimum((np.log(f32(255.0) * op).astype(f32) + f32(0.001)).astype(f32), f32(4.5))
+    hx = (f32(3.0) * np.sqrt(i11 / det).astype(f32) * f32(1.02) + f32(0.02)).astype(f32)
+    s_ = np.sqrt(lim * f32(1.0 / 4.5)).astype(f32)
+    hxl = ((hx - f32(0.02)) * s_ + f32(0.02)).astype(f32)
+    dx = np.nextafter(hxl, np.float32(np.inf)).astype(f32)
+    a2 = (f32(2.0) * i01).astype(f32)
+    for k in (-1, 0, 1):
+        dy = (-(i01.astype(np.float64) * dx) / i11 + k * 1e-3).astype(f32)
+        power = (f32(0.5) * ((i00 * dx * dx + a2 * dx * dy).astype(f32) + (i11 * dy * dy))).astype(f32)
+        assert np.all(power > lim), k
