@@ -171,58 +171,60 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const float dx = pxc - a.x;
       const float txx = a.z * dx * dx;
       const float txy = a.w * dx;
-      float gv[kGradVis];
-#pragma unroll
-      for (int k = 0; k < kGradVis; ++k) gv[k] = 0.0f;
+      float gv[kGradVis];  // set by the first pixel, added to by the others
       bool contrib = false;
 #pragma unroll
       for (int p = 0; p < PIX; ++p) {
-        if (start + j > last[p]) continue;
         const float dy = pyc[p] - a.y;
-        float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
-        bool pclamp = false;
-        if (power < 0.0f) {
-          power = 0.0f;
-          pclamp = true;
-        }
-        if (power > b.z) continue;  // power_limit: the forward skipped it too
-        const float G = splat_exp_neg(-power);
-        float alpha = b.y * G;
-        bool aclamp = false;
-        if (alpha > 0.99f) {
-          alpha = 0.99f;
-          aclamp = true;
-        }
-        if (alpha < kMinAlpha) continue;
-        contrib = true;
-        const float om = 1.0f - alpha;
-        const float rom = __builtin_amdgcn_rcpf(om);
-        const float Tj = T[p] * rom;
-        const float w = alpha * Tj;
-        gv[6] += gC0[p] * w;
-        gv[7] += gC1[p] * w;
-        gv[8] += gC2[p] * w;
-        gv[9] += gD[p] * w;
-        float dal = gC0[p] * (c.x * Tj - SC0[p] * rom) + gC1[p] * (c.y * Tj - SC1[p] * rom) +
-                    gC2[p] * (c.z * Tj - SC2[p] * rom) + gD[p] * (c.w * Tj - SD[p] * rom);
-        if (white) dal -= gsum[p] * Tf[p] * rom;
-        SC0[p] += c.x * w;
-        SC1[p] += c.y * w;
-        SC2[p] += c.z * w;
-        SD[p] += c.w * w;
-        T[p] = Tj;
-        if (!aclamp) {
-          gv[5] += dal * G;
-          if (!pclamp) {
-            const float dp = -dal * b.y * G;
-            const float i01 = 0.5f * a.w;
-            gv[2] += dp * 0.5f * dx * dx;
-            gv[3] += dp * dx * dy;
-            gv[4] += dp * 0.5f * dy * dy;
-            gv[0] += -dp * (a.z * dx + i01 * dy);
-            gv[1] += -dp * (i01 * dx + b.x * dy);
+        // the pixel's blend weight, opacity term and power term, zero where
+        // the splat did not blend (the branches below only set these three,
+        // so the ten gradient components are formed once, after them)
+        float wm = 0.0f, g5 = 0.0f, dpm = 0.0f;
+        if (start + j <= last[p]) {
+          float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
+          bool pclamp = false;
+          if (power < 0.0f) {
+            power = 0.0f;
+            pclamp = true;
+          }
+          if (!(power > b.z)) {  // power_limit: the forward skipped the others too
+            const float G = splat_exp_neg(-power);
+            float alpha = b.y * G;
+            bool aclamp = false;
+            if (alpha > 0.99f) {
+              alpha = 0.99f;
+              aclamp = true;
+            }
+            if (!(alpha < kMinAlpha)) {
+              contrib = true;
+              const float om = 1.0f - alpha;
+              const float rom = __builtin_amdgcn_rcpf(om);
+              const float Tj = T[p] * rom;
+              const float w = alpha * Tj;
+              float dal = gC0[p] * (c.x * Tj - SC0[p] * rom) + gC1[p] * (c.y * Tj - SC1[p] * rom) +
+                          gC2[p] * (c.z * Tj - SC2[p] * rom) + gD[p] * (c.w * Tj - SD[p] * rom);
+              if (white) dal -= gsum[p] * Tf[p] * rom;
+              SC0[p] += c.x * w;
+              SC1[p] += c.y * w;
+              SC2[p] += c.z * w;
+              SD[p] += c.w * w;
+              T[p] = Tj;
+              wm = w;
+              if (!aclamp) {
+                g5 = dal * G;
+                if (!pclamp) dpm = -dal * b.y * G;
+              }
+            }
           }
         }
+        const float i01 = 0.5f * a.w;
+        const float t[kGradVis] = {-dpm * (a.z * dx + i01 * dy), -dpm * (i01 * dx + b.x * dy),
+                                   dpm * 0.5f * dx * dx,         dpm * dx * dy,
+                                   dpm * 0.5f * dy * dy,         g5,
+                                   gC0[p] * wm,                  gC1[p] * wm,
+                                   gC2[p] * wm,                  gD[p] * wm};
+#pragma unroll
+        for (int k = 0; k < kGradVis; ++k) gv[k] = p == 0 ? t[k] : gv[k] + t[k];
       }
       if (__ballot(contrib)) {  // wave-uniform; the whole wave is active here
         float sa, sb, sc;
