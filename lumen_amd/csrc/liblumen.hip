@@ -186,7 +186,12 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict_
 __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
                        const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
                        uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
-                       ViewSpan* __restrict__ spans) {
+                       ViewSpan* __restrict__ spans, int2* __restrict__ ranges,
+                       uint32_t tiles) {
+  // every block: empty tile ranges (-1, -1) before k_ranges fills the used ones
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
+    ranges[t] = make_int2(-1, -1);
+  if (blockIdx.x != 0) return;
   const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
   if (threadIdx.x == 0) {
@@ -203,7 +208,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     plan->dbits = p.dbits;
     plan->m = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
-    if ((uint64_t)M > m_cap || (uint64_t)V > v_cap) plan->overflow = 1u;
+    plan->overflow = ((uint64_t)M > m_cap || (uint64_t)V > v_cap) ? 1u : 0u;
   }
   for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
     const uint64_t a = offsets[(size_t)vw * n];
@@ -751,8 +756,7 @@ static LumenStatus ensure_frame_buffers(LumenExContext* c, uint64_t N, const Fra
   LUMEN_TRY(ensure(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
   LUMEN_TRY(ensure(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 16));
   LUMEN_TRY(ensure(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
-  LUMEN_TRY(ensure(c->totals, 16));
-  LUMEN_TRY(ensure(c->bounds, 16));
+  LUMEN_TRY(ensure(c->totals, 32));  // 8 B totals, then the 16 B bounds (one read-back)
   LUMEN_TRY(ensure(c->plan, sizeof(SortPlan)));
   LUMEN_TRY(ensure(c->ranges, fs.tiles * 8));
   if (save_pixels) {
@@ -774,13 +778,21 @@ static LumenStatus ensure_capacity(LumenExContext* c, uint64_t v_cap, uint64_t m
   return LUMEN_SUCCESS;
 }
 
+// The frame's key bounds (depth key min, max, tile id min, max) sit right
+// after the 8-byte totals, so one copy reads both back.
+static uint32_t* frame_bounds(LumenExContext* c) {
+  return reinterpret_cast<uint32_t*>(c->totals.as<uint8_t>() + 8);
+}
+
 // Stage 1 of a frame: per-(view, Gaussian) counts and their scan.
 static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
                                  hipStream_t st) {
-  hipLaunchKernelGGL(k_init_frame, dim3(1), dim3(1), 0, st, c->bounds.as<uint32_t>(),
-                     c->plan.as<SortPlan>());
-  LUMEN_LAUNCHED();
-  if (sd.n == 0) return LUMEN_SUCCESS;
+  if (sd.n == 0) {  // no k_plan this frame: an empty plan
+    hipLaunchKernelGGL(k_init_frame, dim3(1), dim3(1), 0, st, frame_bounds(c),
+                       c->plan.as<SortPlan>());
+    LUMEN_LAUNCHED();
+    return LUMEN_SUCCESS;
+  }
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
   const uint32_t nblocks = div_up(sd.n, 256);
   hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(256), 0, st, sd, c->views.as<ViewParams>(),
@@ -788,7 +800,7 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
                      c->block_bounds.as<uint4>());
   LUMEN_LAUNCHED();
   hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, st, c->block_bounds.as<uint4>(),
-                     nblocks, c->bounds.as<uint32_t>());
+                     nblocks, frame_bounds(c));
   LUMEN_LAUNCHED();
   const uint64_t nv = (uint64_t)fs.nviews * sd.n;
   LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
@@ -805,14 +817,16 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                                 float* out_depth, hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
-    hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, st, c->packed.as<uint64_t>(),
-                       c->totals.as<uint64_t>(), c->bounds.as<uint32_t>(), sd.n, (int)fs.nviews,
-                       m_cap, v_cap, plan, c->spans.as<ViewSpan>());
+    const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 256 * 16);
+    hipLaunchKernelGGL(k_plan, dim3(pb), dim3(256), 0, st, c->packed.as<uint64_t>(),
+                       c->totals.as<uint64_t>(), frame_bounds(c), sd.n, (int)fs.nviews, m_cap,
+                       v_cap, plan, c->spans.as<ViewSpan>(), c->ranges.as<int2>(),
+                       (uint32_t)fs.tiles);
     LUMEN_LAUNCHED();
   } else {
     LUMEN_TRY(fill<uint32_t>(c->spans.as<uint32_t>(), 0u, 4ull * fs.nviews, st));
+    LUMEN_TRY(fill<int32_t>(c->ranges.as<int32_t>(), -1, 2ull * fs.tiles, st));
   }
-  LUMEN_TRY(fill<int32_t>(c->ranges.as<int32_t>(), -1, 2ull * fs.tiles, st));
   if (sd.n > 0 && m_cap > 0) {
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, false, st));
     hipLaunchKernelGGL(k_emit, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
@@ -886,8 +900,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   uint32_t hb[4] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
   if (sd.n > 0) {
     uint8_t* h = static_cast<uint8_t*>(c->pinned);
-    LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 8, hipMemcpyDeviceToHost, st));
-    LUMEN_HIP(hipMemcpyAsync(h + 8, c->bounds.ptr, 16, hipMemcpyDeviceToHost, st));
+    LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 24, hipMemcpyDeviceToHost, st));  // totals, bounds
     LUMEN_HIP(hipStreamSynchronize(st));
     uint64_t tot;
     memcpy(&tot, h, 8);
@@ -1056,7 +1069,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   graph_release(c->graph);
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->bounds, &c->plan, &c->views, &c->spans,
+  DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->plan, &c->views, &c->spans,
                    &c->geo, &c->block_bounds,
                    &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
                    &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
