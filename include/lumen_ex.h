@@ -94,6 +94,34 @@ LumenExContext* lumen_ex_context_create(int device);
 void lumen_ex_context_destroy(LumenExContext* ctx);
 int lumen_ex_abi_version(void);
 
+/* ---- workspace ----------------------------------------------------------
+ * The reference allocates its buffers per frame (RenderOutput and the
+ * pipeline's lists, src/raster/pipeline.mojo:21-45,141-157); a context keeps
+ * them and grows them on demand instead.  These three calls let a caller size
+ * that workspace up front so that no frame allocates.
+ *
+ * lumen_ex_workspace_size: device bytes a context holds after frames of
+ * n_gaussians splats over these cameras and config with at most
+ * max_instances tile instances per frame (RenderOutput.total_splat_instances,
+ * summed over the views), with the backward and the photometric loss when
+ * with_backward.  config->flags count: LUMEN_EX_SAVE_PIXELS and
+ * LUMEN_EX_KEEP_STAGES add their buffers, LUMEN_EX_GRAPH reserves the graph's
+ * key headroom.  Host-only: touches no device.
+ *
+ * lumen_ex_context_reserve: allocates that workspace in the context; frames
+ * within the bounds then allocate nothing (device or pinned host).
+ *
+ * lumen_ex_context_workspace: device bytes the context holds now. */
+LumenStatus lumen_ex_workspace_size(const LumenExCamera* cameras, uint32_t n_views,
+                                    const LumenExRenderConfig* config, uint64_t n_gaussians,
+                                    uint64_t max_instances, int with_backward,
+                                    size_t* out_bytes);
+LumenStatus lumen_ex_context_reserve(LumenExContext* ctx, const LumenExCamera* cameras,
+                                     uint32_t n_views, const LumenExRenderConfig* config,
+                                     uint64_t n_gaussians, uint64_t max_instances,
+                                     int with_backward);
+LumenStatus lumen_ex_context_workspace(LumenExContext* ctx, size_t* held_bytes);
+
 /* rasterize_scene (src/raster/pipeline.mojo:48-231).  out_rgb: H*W*3 floats,
  * out_depth: H*W floats (device).  Every pixel is written: empty tiles get the
  * background (0, or 1 with white_background) and depth 0.  Blocks the host

@@ -181,6 +181,16 @@ _SIGS = {
                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                                  ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_void_p]),
+    "lumen_ex_workspace_size": (ctypes.c_int, [ctypes.POINTER(LumenExCamera), ctypes.c_uint32,
+                                               ctypes.POINTER(LumenExRenderConfig),
+                                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                               ctypes.POINTER(ctypes.c_size_t)]),
+    "lumen_ex_context_reserve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenExCamera),
+                                                ctypes.c_uint32,
+                                                ctypes.POINTER(LumenExRenderConfig),
+                                                ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int]),
+    "lumen_ex_context_workspace": (ctypes.c_int, [ctypes.c_void_p,
+                                                  ctypes.POINTER(ctypes.c_size_t)]),
     "lumen_ex_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lumen_ex_stage_times": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
 }

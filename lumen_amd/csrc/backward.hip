@@ -40,7 +40,6 @@
 
 namespace lumen {
 
-constexpr int kGradVis = 10;  // mean x, mean y, inv00, inv01, inv11, opacity, r, g, b, depth
 
 __device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ views, int nviews,
                                               uint32_t gt) {
@@ -636,8 +635,8 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   const uint64_t v_cap = c->sizes_known ? c->V : c->v_cap;
   if (sc->count == 0 || v_cap == 0 || (c->sizes_known && c->M == 0)) return LUMEN_SUCCESS;
   const uint64_t m_cap = c->sizes_known ? c->M : c->m_cap;
-  LUMEN_TRY(ensure(c->partial, (m_cap ? m_cap : 1) * kGradVis * sizeof(float)));
-  LUMEN_TRY(ensure(c->grad_vis, v_cap * kGradVis * sizeof(float)));
+  LUMEN_TRY(ensure(c->partial, partial_bytes(m_cap)));
+  LUMEN_TRY(ensure(c->grad_vis, grad_vis_bytes(v_cap)));
   float* gv = c->grad_vis.as<float>();
   const SortPlan* plan = c->plan.as<SortPlan>();
   float* part = c->partial.as<float>();

@@ -52,6 +52,9 @@ struct DevBuf {
   }
 };
 
+// What ensure() allocates for a request of `bytes` (a quarter of headroom).
+inline size_t alloc_size(size_t bytes) { return bytes + bytes / 4 + 256; }
+
 inline LumenStatus ensure(DevBuf& b, size_t bytes) {
   if (bytes <= b.cap && b.ptr) return LUMEN_SUCCESS;
   if (b.ptr) {
@@ -59,7 +62,7 @@ inline LumenStatus ensure(DevBuf& b, size_t bytes) {
     b.ptr = nullptr;
     b.cap = 0;
   }
-  size_t want = bytes + bytes / 4 + 256;
+  size_t want = alloc_size(bytes);
   hipError_t e = hipMalloc(&b.ptr, want);
   if (e != hipSuccess) {
     b.ptr = nullptr;

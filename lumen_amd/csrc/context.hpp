@@ -53,6 +53,30 @@ inline SceneDev scene_dev(const LumenExScene* sc) {
                   (uint32_t)sc->count};
 }
 
+// Backward buffers: per tile instance and per visible splat, the gradients
+// dL/d(mean x, mean y, inv00, inv01, inv11, opacity, r, g, b, depth).
+constexpr int kGradVis = 10;
+inline size_t partial_bytes(uint64_t m_cap) { return (m_cap ? m_cap : 1) * kGradVis * sizeof(float); }
+inline size_t grad_vis_bytes(uint64_t v_cap) { return (v_cap ? v_cap : 1) * kGradVis * sizeof(float); }
+
+// lumen_ex_photometric_loss's partial sums for a width x height image: two
+// floats per block of four 8x8 windows and per block of the pixels outside
+// the windows (at most 2048 such blocks).
+inline uint32_t loss_window_blocks(uint32_t width, uint32_t height) {
+  const uint32_t nwin = (width / 8) * (height / 8);
+  return div_up(nwin ? nwin : 1, 4);
+}
+inline uint32_t loss_rest_blocks(uint32_t width, uint32_t height) {
+  const uint64_t nwin = (uint64_t)(width / 8) * (height / 8);
+  const uint64_t rest = (uint64_t)width * height - nwin * 64u;
+  const uint32_t br = div_up(rest ? rest : 1, 256);
+  return br > 2048 ? 2048 : br;
+}
+inline size_t loss_part_bytes(uint32_t width, uint32_t height) {
+  return (size_t)2 * (loss_window_blocks(width, height) + loss_rest_blocks(width, height)) *
+         sizeof(float);
+}
+
 // Where each view's visible splats and keys sit in the frame's arrays.
 struct ViewSpan {
   uint32_t vis_base, visible, key_base, instances;

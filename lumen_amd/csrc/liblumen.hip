@@ -748,35 +748,55 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t thr
   return LUMEN_SUCCESS;
 }
 
+// The buffers below are named to a `need(DevBuf&, bytes)` callable: ensure()
+// when rendering, a byte count for lumen_ex_workspace_size.
+static LumenStatus need_ensure(DevBuf& b, size_t bytes) { return ensure(b, bytes); }
+
 // Buffers that depend on N, the views and the pixel count.
+template <class Need>
+static LumenStatus frame_buffers(LumenExContext* c, Need&& need, uint64_t N, const FrameShape& fs,
+                                 bool save_pixels) {
+  const uint64_t nv = (uint64_t)fs.nviews * N;
+  LUMEN_TRY(need(c->packed, (nv ? nv : 1) * 8));
+  LUMEN_TRY(need(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
+  LUMEN_TRY(need(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 16));
+  LUMEN_TRY(need(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
+  LUMEN_TRY(need(c->totals, 32));  // 8 B totals, then the 16 B bounds (one read-back)
+  LUMEN_TRY(need(c->plan, sizeof(SortPlan)));
+  LUMEN_TRY(need(c->ranges, fs.tiles * 8));
+  if (save_pixels) {
+    LUMEN_TRY(need(c->t_final, fs.pixels * 4));
+    LUMEN_TRY(need(c->n_eval, fs.pixels * 4));
+  }
+  return LUMEN_SUCCESS;
+}
+
 static LumenStatus ensure_frame_buffers(LumenExContext* c, uint64_t N, const FrameShape& fs,
                                         bool save_pixels) {
-  const uint64_t nv = (uint64_t)fs.nviews * N;
-  LUMEN_TRY(ensure(c->packed, (nv ? nv : 1) * 8));
-  LUMEN_TRY(ensure(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
-  LUMEN_TRY(ensure(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 16));
-  LUMEN_TRY(ensure(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
-  LUMEN_TRY(ensure(c->totals, 32));  // 8 B totals, then the 16 B bounds (one read-back)
-  LUMEN_TRY(ensure(c->plan, sizeof(SortPlan)));
-  LUMEN_TRY(ensure(c->ranges, fs.tiles * 8));
-  if (save_pixels) {
-    LUMEN_TRY(ensure(c->t_final, fs.pixels * 4));
-    LUMEN_TRY(ensure(c->n_eval, fs.pixels * 4));
+  return frame_buffers(c, need_ensure, N, fs, save_pixels);
+}
+
+// Buffers sized by the frame's visible splats and tile instances.
+template <class Need>
+static LumenStatus capacity_buffers(LumenExContext* c, Need&& need, uint64_t v_cap,
+                                    uint64_t m_cap) {
+  LUMEN_TRY(need(c->rec, (v_cap ? v_cap : 1) * 48));
+  LUMEN_TRY(need(c->vis2orig, (v_cap ? v_cap : 1) * 4));
+  LUMEN_TRY(need(c->radius, (v_cap ? v_cap : 1) * 4));
+  LUMEN_TRY(need(c->vmeta, (v_cap ? v_cap : 1) * 16));
+  for (int b = 0; b < 2; ++b) {
+    LUMEN_TRY(need(c->keys[b], (m_cap ? m_cap : 1) * 8));
+    LUMEN_TRY(need(c->vals[b], (m_cap ? m_cap : 1) * 4));
   }
   return LUMEN_SUCCESS;
 }
 
 static LumenStatus ensure_capacity(LumenExContext* c, uint64_t v_cap, uint64_t m_cap) {
-  LUMEN_TRY(ensure(c->rec, (v_cap ? v_cap : 1) * 48));
-  LUMEN_TRY(ensure(c->vis2orig, (v_cap ? v_cap : 1) * 4));
-  LUMEN_TRY(ensure(c->radius, (v_cap ? v_cap : 1) * 4));
-  LUMEN_TRY(ensure(c->vmeta, (v_cap ? v_cap : 1) * 16));
-  for (int b = 0; b < 2; ++b) {
-    LUMEN_TRY(ensure(c->keys[b], (m_cap ? m_cap : 1) * 8));
-    LUMEN_TRY(ensure(c->vals[b], (m_cap ? m_cap : 1) * 4));
-  }
-  return LUMEN_SUCCESS;
+  return capacity_buffers(c, need_ensure, v_cap, m_cap);
 }
+
+// Key capacity a graph frame reserves over the instances of its first frame.
+static uint64_t graph_m_cap(uint64_t m) { return m + m / 4 + 65536; }
 
 // The frame's key bounds (depth key min, max, tile id min, max) sit right
 // after the 8-byte totals, so one copy reads both back.
@@ -987,18 +1007,14 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   LumenExRenderConfig pc = *cfg;
   pc.flags &= ~LUMEN_EX_KEEP_STAGES;
   LUMEN_TRY(render_plain(c, sc, cams, nviews, &pc, out_rgb, out_depth, nullptr, st));
-  const uint64_t m_cap = c->M + c->M / 4 + 65536;
+  const uint64_t m_cap = graph_m_cap(c->M);
   const uint64_t v_cap = (uint64_t)nviews * sc->count;
   LUMEN_TRY(ensure_capacity(c, v_cap, m_cap));
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
   // sort scratch for the capacity, allocated before capture
-  {
-    const uint32_t nb = div_up(m_cap, kSortTile);
-    const uint64_t hlen = (uint64_t)nb * kDigitBins;
-    LUMEN_TRY(ensure(c->hist, hlen * sizeof(uint32_t)));
-    LUMEN_TRY(ensure(c->hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
-  }
+  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_cap)));
   LUMEN_HIP(hipStreamSynchronize(st));
   const int slots = max_sort_slots(c, fs);
   const SceneDev sd = scene_dev(sc);
@@ -1032,6 +1048,82 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   c->graph.m_cap = m_cap;
   c->v_cap = v_cap;
   c->m_cap = m_cap;
+  return LUMEN_SUCCESS;
+}
+
+// ---- workspace ------------------------------------------------------------
+
+// Every device buffer of a context.
+static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
+  return {&c->packed, &c->part,     &c->totals,    &c->plan,      &c->views,    &c->spans,
+          &c->geo,    &c->block_bounds, &c->rec,   &c->vis2orig,  &c->radius,   &c->vmeta,
+          &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
+          &c->hist,   &c->hist_part, &c->ranges,   &c->t_final,   &c->n_eval,   &c->minmax,
+          &c->partial, &c->grad_vis, &c->loss_part};
+}
+
+struct WorkspaceShape {
+  FrameShape fs;
+  uint32_t max_w = 0, max_h = 0;  // the largest view (the loss's partial sums)
+};
+
+static LumenStatus workspace_shape(const LumenExCamera* cams, uint32_t n,
+                                   const LumenExRenderConfig* cfg, WorkspaceShape& ws) {
+  uint64_t tb = 0, pb = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint64_t tx = (cams[k].width + cfg->tile_width - 1) / cfg->tile_width;
+    const uint64_t ty = (cams[k].height + cfg->tile_height - 1) / cfg->tile_height;
+    tb += tx * ty;
+    pb += (uint64_t)cams[k].width * cams[k].height;
+    if ((uint64_t)cams[k].width * cams[k].height > (uint64_t)ws.max_w * ws.max_h) {
+      ws.max_w = cams[k].width;
+      ws.max_h = cams[k].height;
+    }
+  }
+  if (tb >= 0x7FFFFFFFull) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "too many tiles in the frame");
+  ws.fs.nviews = n;
+  ws.fs.tiles = tb;
+  ws.fs.pixels = pb;
+  return LUMEN_SUCCESS;
+}
+
+// Every buffer that frames of this shape use with at most `max_instances`
+// tile instances (and the backward and its loss when `backward`), named to
+// `need`: the sizes render_plain, render_graph, the sort and the backward
+// ask for, at their largest.
+template <class Need>
+static LumenStatus workspace(LumenExContext* c, Need&& need, uint64_t N, const WorkspaceShape& ws,
+                             const LumenExRenderConfig* cfg, uint64_t max_instances,
+                             bool backward) {
+  const uint32_t flags = cfg->flags;
+  const uint64_t v_cap = (uint64_t)ws.fs.nviews * N;
+  const uint64_t m_cap = (flags & LUMEN_EX_GRAPH) ? graph_m_cap(max_instances) : max_instances;
+  const bool save = backward || (flags & LUMEN_EX_SAVE_PIXELS);
+  LUMEN_TRY(need(c->views, sizeof(ViewParams) * ws.fs.nviews));
+  LUMEN_TRY(need(c->spans, sizeof(ViewSpan) * ws.fs.nviews));
+  LUMEN_TRY(frame_buffers(c, need, N, ws.fs, save));
+  LUMEN_TRY(capacity_buffers(c, need, v_cap, m_cap));
+  if (m_cap > 0) {
+    LUMEN_TRY(need(c->hist, sort_hist_len(m_cap) * sizeof(uint32_t)));
+    LUMEN_TRY(need(c->hist_part, sort_part_bytes(m_cap)));
+  }
+  if ((flags & LUMEN_EX_KEEP_STAGES) && m_cap > 0) {
+    LUMEN_TRY(need(c->keys_keep, m_cap * 8));
+    LUMEN_TRY(need(c->vals_keep, m_cap * 4));
+  }
+  if (backward) {
+    LUMEN_TRY(need(c->partial, partial_bytes(m_cap)));
+    LUMEN_TRY(need(c->grad_vis, grad_vis_bytes(v_cap)));
+    LUMEN_TRY(need(c->loss_part, loss_part_bytes(ws.max_w, ws.max_h)));
+  }
+  return LUMEN_SUCCESS;
+}
+
+static LumenStatus check_workspace_args(const LumenExCamera* cams, uint32_t n,
+                                        const LumenExRenderConfig* cfg, const char* who) {
+  if (!cams || !cfg || n == 0) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "%s: null argument", who);
+  LUMEN_TRY(check_config(cfg));
+  LUMEN_TRY(check_cameras(cams, n));
   return LUMEN_SUCCESS;
 }
 
@@ -1069,16 +1161,75 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   graph_release(c->graph);
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  DevBuf* all[] = {&c->packed, &c->part, &c->totals, &c->plan, &c->views, &c->spans,
-                   &c->geo, &c->block_bounds,
-                   &c->rec, &c->vis2orig, &c->radius, &c->keys[0], &c->keys[1], &c->vals[0],
-                   &c->vals[1], &c->keys_keep, &c->vals_keep, &c->hist, &c->hist_part,
-                   &c->ranges, &c->t_final, &c->n_eval, &c->minmax, &c->grad_vis,
-                   &c->vmeta, &c->partial, &c->loss_part};
-  for (DevBuf* b : all) release(*b);
+  for (DevBuf* b : context_buffers(c)) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);
   delete c;
+}
+
+LumenStatus lumen_ex_workspace_size(const LumenExCamera* cameras, uint32_t n_views,
+                                    const LumenExRenderConfig* config, uint64_t n_gaussians,
+                                    uint64_t max_instances, int with_backward,
+                                    size_t* out_bytes) {
+  clear_error();
+  LUMEN_TRY(check_workspace_args(cameras, n_views, config, "lumen_ex_workspace_size"));
+  if (!out_bytes) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_workspace_size: null argument");
+  WorkspaceShape ws;
+  LUMEN_TRY(workspace_shape(cameras, n_views, config, ws));
+  // a context that is never given to the device: only its buffers' names
+  LumenExContext sizing;
+  std::vector<std::pair<DevBuf*, size_t>> asked;
+  auto count = [&asked](DevBuf& b, size_t bytes) -> LumenStatus {
+    for (auto& a : asked)
+      if (a.first == &b) {
+        a.second = bytes > a.second ? bytes : a.second;
+        return LUMEN_SUCCESS;
+      }
+    asked.emplace_back(&b, bytes);
+    return LUMEN_SUCCESS;
+  };
+  LUMEN_TRY(workspace(&sizing, count, n_gaussians, ws, config, max_instances, with_backward != 0));
+  size_t total = 0;
+  for (auto& a : asked) total += alloc_size(a.second);
+  *out_bytes = total;
+  return LUMEN_SUCCESS;
+}
+
+LumenStatus lumen_ex_context_reserve(LumenExContext* c, const LumenExCamera* cameras,
+                                     uint32_t n_views, const LumenExRenderConfig* config,
+                                     uint64_t n_gaussians, uint64_t max_instances,
+                                     int with_backward) {
+  clear_error();
+  if (!c) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_context_reserve: null argument");
+  LUMEN_TRY(check_workspace_args(cameras, n_views, config, "lumen_ex_context_reserve"));
+  WorkspaceShape ws;
+  LUMEN_TRY(workspace_shape(cameras, n_views, config, ws));
+  LUMEN_HIP(hipSetDevice(c->device));
+  if (c->graph.valid) graph_release(c->graph);  // a growing buffer would move under it
+  LUMEN_TRY(workspace(c, need_ensure, n_gaussians, ws, config, max_instances, with_backward != 0));
+  // the host staging of the per-view parameters
+  const size_t vbytes = sizeof(ViewParams) * n_views;
+  if (c->pinned_views_cap < vbytes) {
+    if (c->pinned_views) {
+      LUMEN_HIP(hipDeviceSynchronize());
+      LUMEN_HIP(hipHostFree(c->pinned_views));
+    }
+    c->pinned_views = nullptr;
+    c->pinned_views_cap = 0;
+    LUMEN_HIP(hipHostMalloc(&c->pinned_views, vbytes, hipHostMallocDefault));
+    c->pinned_views_cap = vbytes;
+  }
+  return LUMEN_SUCCESS;
+}
+
+LumenStatus lumen_ex_context_workspace(LumenExContext* c, size_t* held_bytes) {
+  clear_error();
+  if (!c || !held_bytes)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_context_workspace: null argument");
+  size_t total = 0;
+  for (DevBuf* b : context_buffers(c)) total += b->ptr ? b->cap : 0;
+  *held_bytes = total;
+  return LUMEN_SUCCESS;
 }
 
 LumenStatus lumen_ex_render(LumenExContext* c, const LumenExScene* sc, const LumenExCamera* cam,

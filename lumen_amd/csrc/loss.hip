@@ -209,11 +209,9 @@ extern "C" LumenStatus lumen_ex_photometric_loss(LumenExContext* c, const float*
   const uint32_t nwin = sh.wx * sh.hy;
   sh.w_l1 = 1.0f - lambda_dssim;
   sh.w_ssim = nwin ? lambda_dssim / (float)nwin : 0.0f;
-  const uint32_t bw = div_up(nwin ? nwin : 1, 4);
-  const uint64_t rest = (uint64_t)width * height - (uint64_t)nwin * 64u;
-  uint32_t br = div_up(rest ? rest : 1, 256);
-  if (br > 2048) br = 2048;
-  LUMEN_TRY(ensure(c->loss_part, (size_t)2 * (bw + br) * sizeof(float)));
+  const uint32_t bw = loss_window_blocks(width, height);
+  const uint32_t br = loss_rest_blocks(width, height);
+  LUMEN_TRY(ensure(c->loss_part, loss_part_bytes(width, height)));
   float* pw = c->loss_part.as<float>();
   float* pr = pw + 2u * bw;
   hipLaunchKernelGGL(k_loss_windows, dim3(bw), dim3(256), 0, st, pred, target, sh, grad, pw);

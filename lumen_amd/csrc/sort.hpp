@@ -190,6 +190,15 @@ struct SortScratch {
   DevBuf* hist_part;
 };
 
+// Histogram rows (one per digit per block) and the scan's partial sums for
+// sorting up to m_cap keys.
+inline uint64_t sort_hist_len(uint64_t m_cap) {
+  return (uint64_t)div_up(m_cap, kSortTile) * kDigitBins;
+}
+inline size_t sort_part_bytes(uint64_t m_cap) {
+  return (size_t)div_up(sort_hist_len(m_cap), kScanChunk) * sizeof(uint32_t);
+}
+
 // Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
 // memory; the result ends in buffer (plan->passes & 1), as 64-bit keys
 // (mode 0) or 32-bit frame keys (mode 1) in the same buffers.  `modes` says
@@ -201,9 +210,9 @@ inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const Sort
                                       hipStream_t st) {
   if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
   const uint32_t nb = div_up(m_cap, kSortTile);
-  const uint64_t hlen = (uint64_t)nb * kDigitBins;
+  const uint64_t hlen = sort_hist_len(m_cap);
   LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
-  LUMEN_TRY(ensure(*sc.hist_part, (uint64_t)div_up(hlen, kScanChunk) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap)));
   uint32_t* hist = sc.hist->as<uint32_t>();
   uint32_t* part = sc.hist_part->as<uint32_t>();
   uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};

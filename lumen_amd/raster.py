@@ -156,6 +156,15 @@ class Rasterizer:
         _lib.check("lumen_ex_copy_stage", st)
         return out
 
+    def frame_stage(self, name: str, count: int, dtype) -> np.ndarray:
+        """Host copy of `count` elements of a stage buffer of the last frame
+        (single or multi-view) on this context."""
+        out = np.empty(count, dtype)
+        st = _lib.lib().lumen_ex_copy_stage(self._ctx, _lib.STAGES[name],
+                                            out.ctypes.data_as(ctypes.c_void_p), out.nbytes)
+        _lib.check("lumen_ex_copy_stage", st)
+        return out
+
     def render_backward(self, scene, camera: PinholeCamera, config: RenderConfig, grad_rgb,
                         grad_depth, grads: "GaussianGradients") -> None:
         """Accumulate the gradient of the last render (made with save_pixels=True
@@ -170,6 +179,24 @@ class Rasterizer:
                                                  ctypes.byref(cfg), grad_rgb.data_ptr(), gd,
                                                  ctypes.byref(g), _stream_handle(scene.device))
         _lib.check("lumen_ex_render_backward", st)
+
+    def reserve(self, n_gaussians: int, cameras, config: Optional[RenderConfig] = None,
+                max_instances: int = 0, backward: bool = False, save_pixels: bool = False,
+                graph: bool = False) -> None:
+        """Allocate the workspace of frames up to these bounds now, so that
+        those frames allocate nothing (lumen_ex_context_reserve)."""
+        cams, n, cfg = _workspace_args(cameras, config, save_pixels, graph)
+        st = _lib.lib().lumen_ex_context_reserve(self._ctx, cams, n, ctypes.byref(cfg),
+                                                 int(n_gaussians), int(max_instances),
+                                                 int(bool(backward)))
+        _lib.check("lumen_ex_context_reserve", st)
+
+    def workspace_bytes(self) -> int:
+        """Device bytes the context holds (lumen_ex_context_workspace)."""
+        out = ctypes.c_size_t(0)
+        _lib.check("lumen_ex_context_workspace",
+                   _lib.lib().lumen_ex_context_workspace(self._ctx, ctypes.byref(out)))
+        return int(out.value)
 
     def set_profiling(self, enable: bool) -> None:
         _lib.check("lumen_ex_set_profiling",
@@ -190,6 +217,30 @@ class Rasterizer:
         st = _lib.lib().lumen_ex_radix_sort(self._ctx, keys.data_ptr(), values.data_ptr(),
                                             keys.numel(), _stream_handle(keys.device))
         _lib.check("lumen_ex_radix_sort", st)
+
+
+def _workspace_args(cameras, config, save_pixels, graph):
+    if isinstance(cameras, PinholeCamera):
+        cameras = [cameras]
+    n = len(cameras)
+    cams = (_lib.LumenExCamera * n)(*[camera_struct(c) for c in cameras])
+    flags = (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0) | (_lib.LUMEN_EX_GRAPH if graph else 0)
+    return cams, n, (config or RenderConfig()).ex_struct(flags)
+
+
+def workspace_size(n_gaussians: int, cameras, config: Optional[RenderConfig] = None,
+                   max_instances: int = 0, backward: bool = False, save_pixels: bool = False,
+                   graph: bool = False) -> int:
+    """Device bytes a context holds for frames of n_gaussians splats over these
+    cameras with at most max_instances tile instances (lumen_ex_workspace_size;
+    host-only)."""
+    cams, n, cfg = _workspace_args(cameras, config, save_pixels, graph)
+    out = ctypes.c_size_t(0)
+    st = _lib.lib().lumen_ex_workspace_size(cams, n, ctypes.byref(cfg), int(n_gaussians),
+                                            int(max_instances), int(bool(backward)),
+                                            ctypes.byref(out))
+    _lib.check("lumen_ex_workspace_size", st)
+    return int(out.value)
 
 
 def build_ranges(sorted_keys, tiles: int):

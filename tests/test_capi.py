@@ -174,3 +174,26 @@ def test_synthetic_sphere_matches_reference_library(L, n, radius):
     np.testing.assert_array_equal(A, B)
     L.lumen_cloud_destroy(a)
     R.lumen_cloud_destroy(b)
+
+
+def test_workspace_size_without_gpu():
+    """lumen_ex_workspace_size is host arithmetic: it grows with the splats,
+    instances, views and the backward, and rejects what lumen_ex_render would."""
+    from lumen_amd._lib import LumenError
+    from lumen_amd.camera import CameraConfig, look_at
+    from lumen_amd.raster import RenderConfig, workspace_size
+    cam = look_at(CameraConfig(1920, 1080, 1400.0, 1400.0, 960.0, 540.0), (0.0, 0.0, 5.0))
+    base = workspace_size(1_000_000, cam, max_instances=6_000_000)
+    # at least the ping-pong keys and values and the 48-byte records
+    assert base >= 2 * 12 * 6_000_000 + 48 * 1_000_000
+    assert workspace_size(2_000_000, cam, max_instances=6_000_000) > base
+    assert workspace_size(1_000_000, cam, max_instances=12_000_000) > base
+    assert workspace_size(1_000_000, [cam, cam], max_instances=6_000_000) > base
+    bwd = workspace_size(1_000_000, cam, max_instances=6_000_000, backward=True)
+    assert bwd >= base + 40 * 6_000_000
+    assert workspace_size(1_000_000, cam, max_instances=6_000_000, graph=True) > base
+    assert workspace_size(0, cam) > 0
+    with pytest.raises(LumenError):
+        workspace_size(10, cam, RenderConfig(tile_width=10, tile_height=10))
+    with pytest.raises(LumenError):
+        workspace_size(10, [])
