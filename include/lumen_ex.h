@@ -257,6 +257,91 @@ LumenStatus lumen_ex_photometric_loss(LumenExContext* ctx, const float* pred, co
                                       uint32_t width, uint32_t height, float lambda_dssim,
                                       float* out, float* grad, void* stream);
 
+/* ---- the optimizer side of a training step --------------------------------
+ * Mutable Gaussian planes in the GaussianSoA layout (the scene itself, or a
+ * per-Gaussian state such as Adam's moments): 11 planes of count floats and
+ * sh = count*48 floats. */
+typedef struct {
+    float* pos_x;
+    float* pos_y;
+    float* pos_z;
+    float* scale_x;
+    float* scale_y;
+    float* scale_z;
+    float* rot_w;
+    float* rot_x;
+    float* rot_y;
+    float* rot_z;
+    float* opacity;
+    float* sh;
+} LumenExPlanes;
+
+/* TrainingConfig fields the optimizer and density control read
+ * (src/core/config.mojo:113-166) and the Adam constants
+ * (src/optim/optimizer.mojo:80-83).  Reference defaults in comments. */
+typedef struct {
+    int64_t iterations;              /* 30000 */
+    float position_lr_init;          /* 0.00016 */
+    float position_lr_final;         /* 0.0000016 */
+    float feature_lr;                /* 0.0025 */
+    float opacity_lr;                /* 0.05 */
+    float scaling_lr;                /* 0.005 */
+    float rotation_lr;               /* 0.001 */
+    float beta1;                     /* 0.9 */
+    float beta2;                     /* 0.999 */
+    float epsilon;                   /* 1e-8 */
+    float densify_grad_threshold;    /* 0.0002 */
+    float prune_opacity_threshold;   /* 0.005 */
+} LumenExTrainConfig;
+
+#define LUMEN_EX_ADAM_ALL_PARAMS 1u  /* also scale, rotation and SH (see below) */
+
+/* AdamOptimizer.step (src/optim/optimizer.mojo:112-170) for the step count
+ * `step` (>= 1, the reference's step_count after its increment): pos x, y, z
+ * with get_position_lr(step) (optimizer.mojo:103-110) and opacity with
+ * opacity_lr, clamped to [0.01, 0.99], in place on device planes; m and v
+ * are the first and second moments (zero before the first step).  Only the
+ * pos and opacity planes of grads, m and v are read unless
+ * LUMEN_EX_ADAM_ALL_PARAMS, which also updates scale (scaling_lr), rotation
+ * (rotation_lr) and SH (feature_lr) the same way without a clamp (the
+ * reference declares these rates but does not apply them).  The same
+ * result, bit for bit, as the reference's scalar loop. */
+LumenStatus lumen_ex_adam_step(const LumenExPlanes* params, const LumenExGradients* grads,
+                               const LumenExPlanes* m, const LumenExPlanes* v, uint64_t count,
+                               const LumenExTrainConfig* config, int64_t step, uint32_t flags,
+                               void* stream);
+
+/* reset_opacities (src/optim/density_control.mojo:101-109): every opacity
+ * above max_value becomes max_value (device plane, in place). */
+LumenStatus lumen_ex_reset_opacities(float* opacity, uint64_t count, float max_value,
+                                     void* stream);
+
+typedef struct {
+    uint64_t cloned;  /* DensityControlStats (density_control.mojo:14-22) */
+    uint64_t split;
+    uint64_t pruned;
+    uint64_t count;   /* Gaussians written to out */
+} LumenExDensifyStats;
+
+/* densify_and_prune (src/optim/density_control.mojo:25-98) from the scene
+ * and its view-space gradient statistics (GaussianGradients'
+ * viewspace_grad_2d and observation_count) into the caller's `out` planes,
+ * which hold out_capacity Gaussians (2 * count always suffices).  Output in
+ * the reference's order, rotations normalized as its Gaussian3D constructor
+ * does.  moments_in / moments_out (both NULL, or each an array {m, v} of
+ * full LumenExPlanes) carry Adam's state over: kept Gaussians and a clone's
+ * original keep their moments, the new ones start at zero.  Blocks the
+ * host once (the new count); returns LUMEN_EX_ERROR_CAPACITY with stats
+ * filled and nothing written when the result exceeds out_capacity. */
+LumenStatus lumen_ex_densify_and_prune(LumenExContext* ctx, const LumenExScene* scene,
+                                       const float* viewspace_grad_2d,
+                                       const int32_t* observation_count,
+                                       const LumenExTrainConfig* config, float scene_extent,
+                                       const LumenExPlanes* out, uint64_t out_capacity,
+                                       const LumenExPlanes* moments_in,
+                                       const LumenExPlanes* moments_out,
+                                       LumenExDensifyStats* stats, void* stream);
+
 /* Per-stage GPU timing with HIP events recorded on the render's stream around
  * each stage of lumen_ex_render (off by default).  lumen_ex_stage_times waits
  * for the last render's events and writes up to n durations in ms, in the

@@ -195,6 +195,44 @@ _SIGS = {
     "lumen_ex_stage_times": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_int]),
 }
 
+class LumenExPlanes(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in (
+        "pos_x", "pos_y", "pos_z", "scale_x", "scale_y", "scale_z", "rot_w", "rot_x", "rot_y",
+        "rot_z", "opacity", "sh")]
+
+
+class LumenExTrainConfig(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64)] + [(k, ctypes.c_float) for k in (
+        "position_lr_init", "position_lr_final", "feature_lr", "opacity_lr", "scaling_lr",
+        "rotation_lr", "beta1", "beta2", "epsilon", "densify_grad_threshold",
+        "prune_opacity_threshold")]
+
+
+class LumenExDensifyStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("cloned", "split", "pruned", "count")]
+
+
+LUMEN_EX_ADAM_ALL_PARAMS = 1
+
+_SIGS.update({
+    "lumen_ex_adam_step": (ctypes.c_int, [ctypes.POINTER(LumenExPlanes),
+                                          ctypes.POINTER(LumenExGradients),
+                                          ctypes.POINTER(LumenExPlanes),
+                                          ctypes.POINTER(LumenExPlanes), ctypes.c_uint64,
+                                          ctypes.POINTER(LumenExTrainConfig), ctypes.c_int64,
+                                          ctypes.c_uint32, ctypes.c_void_p]),
+    "lumen_ex_reset_opacities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_float,
+                                                ctypes.c_void_p]),
+    "lumen_ex_densify_and_prune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenExScene),
+                                                  ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.POINTER(LumenExTrainConfig),
+                                                  ctypes.c_float, ctypes.POINTER(LumenExPlanes),
+                                                  ctypes.c_uint64, ctypes.c_void_p,
+                                                  ctypes.c_void_p,
+                                                  ctypes.POINTER(LumenExDensifyStats),
+                                                  ctypes.c_void_p]),
+})
+
 TIMED_STAGES = ("count", "emit", "sort", "ranges", "composite", "bwd_composite", "bwd_preprocess")
 
 _lib = None

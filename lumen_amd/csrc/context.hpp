@@ -104,6 +104,9 @@ struct LumenExContext {
   lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
   lumen::DevBuf loss_part; // lumen_ex_photometric_loss: per-block partial sums
+  // lumen_ex_densify_and_prune: per-Gaussian class, output count and offset;
+  // stats, total and the scan's partial sums
+  lumen::DevBuf dens_cls, dens_cnt, dens_off, dens_misc;
   void* pinned = nullptr;       // small host staging (totals, bounds, plan)
   void* pinned_views = nullptr; // host staging of the per-view parameters
   size_t pinned_views_cap = 0;

@@ -1059,7 +1059,8 @@ static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
           &c->geo,    &c->block_bounds, &c->rec,   &c->vis2orig,  &c->radius,   &c->vmeta,
           &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
           &c->hist,   &c->hist_part, &c->ranges,   &c->t_final,   &c->n_eval,   &c->minmax,
-          &c->partial, &c->grad_vis, &c->loss_part};
+          &c->partial, &c->grad_vis, &c->loss_part, &c->dens_cls, &c->dens_cnt, &c->dens_off,
+          &c->dens_misc};
 }
 
 struct WorkspaceShape {

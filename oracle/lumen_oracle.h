@@ -139,6 +139,31 @@ int orc64_backward(const orc_scene64 *sc, const orc_camera *cam, const orc_confi
                    double *g_vis);
 
 /* ---- losses (src/optim/loss.mojo) -------------------------------------- */
+/* ---- optimizer and density control (lumen_oracle_optim.c) ------------- */
+
+/* TrainingConfig fields these read (src/core/config.mojo:113-166) and the
+ * Adam constants (src/optim/optimizer.mojo:80-83). */
+typedef struct {
+    int64_t iterations;
+    float position_lr_init, position_lr_final, feature_lr, opacity_lr, scaling_lr, rotation_lr;
+    float beta1, beta2, epsilon;
+    float densify_grad_threshold, prune_opacity_threshold;
+} orc_train_config;
+
+typedef struct {
+    float pos_lr, inv_b1, inv_b2;
+} orc_adam_scalars;
+
+void orc_adam_schedule(const orc_train_config *cfg, int64_t step, orc_adam_scalars *out);
+void orc_adam_step(float *planes, float *sh, const float *gplanes, const float *gsh, float *mplanes,
+                   float *msh, float *vplanes, float *vsh, int64_t n, const orc_train_config *c,
+                   const orc_adam_scalars *s, int all_params);
+void orc_reset_opacities(float *opacity, int64_t n, float max_val);
+int64_t orc_densify_and_prune(const float *planes, const float *sh, const float *vs_grad,
+                              const int32_t *vs_count, int64_t n, const orc_train_config *c,
+                              float scene_extent, float *out_planes, float *out_sh, int64_t cap,
+                              int64_t *origin, int64_t stats[3]);
+
 float orc_l1_loss(const float *pred, const float *target, int64_t n, float *grad);
 float orc_ssim(const float *pred, const float *target, int64_t w, int64_t h, int64_t win);
 float orc_psnr(const float *pred, const float *target, int64_t n);

@@ -288,3 +288,98 @@ def backward64(planes, sh, camera, config, g_rgb, g_depth):
                          g_depth.ctypes.data_as(f64p), ctypes.byref(gr),
                          gvis.ctypes.data_as(f64p))
     return {"planes": gp, "sh": gsh, "viewspace": vs, "count": cnt, "per_visible": gvis[:V]}
+
+
+# ---- optimizer and density control (lumen_oracle_optim.c) --------------------
+
+class OrcTrainConfig(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64)] + [(k, ctypes.c_float) for k in (
+        "position_lr_init", "position_lr_final", "feature_lr", "opacity_lr", "scaling_lr",
+        "rotation_lr", "beta1", "beta2", "epsilon", "densify_grad_threshold",
+        "prune_opacity_threshold")]
+
+
+class OrcAdamScalars(ctypes.Structure):
+    _fields_ = [("pos_lr", ctypes.c_float), ("inv_b1", ctypes.c_float),
+                ("inv_b2", ctypes.c_float)]
+
+
+def train_config(cfg=None, **kw) -> OrcTrainConfig:
+    """OrcTrainConfig from an object with TrainingConfig's fields (or the
+    reference defaults) and keyword overrides."""
+    d = dict(iterations=30000, position_lr_init=0.00016, position_lr_final=0.0000016,
+             feature_lr=0.0025, opacity_lr=0.05, scaling_lr=0.005, rotation_lr=0.001, beta1=0.9,
+             beta2=0.999, epsilon=1e-8, densify_grad_threshold=0.0002,
+             prune_opacity_threshold=0.005)
+    if cfg is not None:
+        for k in d:
+            if hasattr(cfg, k):
+                d[k] = getattr(cfg, k)
+    d.update(kw)
+    c = OrcTrainConfig()
+    for k, v in d.items():
+        setattr(c, k, v)
+    return c
+
+
+def _optim_lib():
+    L = lib()
+    if not getattr(L, "_optim_ready", False):
+        L.orc_adam_schedule.restype = None
+        L.orc_adam_schedule.argtypes = [ctypes.POINTER(OrcTrainConfig), ctypes.c_int64,
+                                        ctypes.POINTER(OrcAdamScalars)]
+        L.orc_adam_step.restype = None
+        L.orc_adam_step.argtypes = [f32p] * 8 + [ctypes.c_int64, ctypes.POINTER(OrcTrainConfig),
+                                                 ctypes.POINTER(OrcAdamScalars), ctypes.c_int]
+        L.orc_reset_opacities.restype = None
+        L.orc_reset_opacities.argtypes = [f32p, ctypes.c_int64, ctypes.c_float]
+        L.orc_densify_and_prune.restype = ctypes.c_int64
+        L.orc_densify_and_prune.argtypes = [f32p, f32p, f32p, i32p, ctypes.c_int64,
+                                            ctypes.POINTER(OrcTrainConfig), ctypes.c_float, f32p,
+                                            f32p, ctypes.c_int64, i64p, i64p]
+        L._optim_ready = True
+    return L
+
+
+def adam_schedule(cfg: OrcTrainConfig, step: int) -> OrcAdamScalars:
+    s = OrcAdamScalars()
+    _optim_lib().orc_adam_schedule(ctypes.byref(cfg), step, ctypes.byref(s))
+    return s
+
+
+def adam_step(planes, sh, gplanes, gsh, m, v, cfg: OrcTrainConfig, step: int,
+              all_params: bool = False) -> None:
+    """In place on float32 numpy arrays: planes (11, n), sh (n, 48), and the
+    moments m = (m_planes, m_sh), v = (v_planes, v_sh)."""
+    arrs = [planes, sh, gplanes, gsh, m[0], m[1], v[0], v[1]]
+    for a in arrs:
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    s = adam_schedule(cfg, step)
+    _optim_lib().orc_adam_step(*[fptr(a) for a in arrs], planes.shape[1], ctypes.byref(cfg),
+                               ctypes.byref(s), int(bool(all_params)))
+
+
+def reset_opacities(opacity: np.ndarray, max_val: float) -> None:
+    assert opacity.dtype == np.float32 and opacity.flags.c_contiguous
+    _optim_lib().orc_reset_opacities(fptr(opacity), opacity.size, max_val)
+
+
+def densify_and_prune(planes, sh, vs_grad, vs_count, cfg: OrcTrainConfig, scene_extent: float):
+    """(planes (11, m), sh (m, 48), origin (m,), (cloned, split, pruned))."""
+    n = planes.shape[1]
+    cap = max(2 * n, 1)
+    op = np.zeros((11, cap), np.float32)
+    osh = np.zeros((cap, 48), np.float32)
+    origin = np.zeros(cap, np.int64)
+    stats = np.zeros(3, np.int64)
+    planes = np.ascontiguousarray(planes, np.float32)
+    sh = np.ascontiguousarray(sh, np.float32)
+    vs_grad = np.ascontiguousarray(vs_grad, np.float32)
+    vs_count = np.ascontiguousarray(vs_count, np.int32)
+    m = _optim_lib().orc_densify_and_prune(
+        fptr(planes), fptr(sh), fptr(vs_grad), vs_count.ctypes.data_as(i32p), n,
+        ctypes.byref(cfg), scene_extent, fptr(op), fptr(osh), cap,
+        origin.ctypes.data_as(i64p), stats.ctypes.data_as(i64p))
+    assert m >= 0
+    return (np.ascontiguousarray(op[:, :m]), osh[:m].copy(), origin[:m].copy(),
+            tuple(int(x) for x in stats))
