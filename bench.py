@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the splat rasterization path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--legs fwd_bwd,batch,serve]
+    python bench.py [--steps K] [--warmup W] [--legs fwd_bwd,batch,serve,optim]
                     [--no-cpu-baseline] [--copy-calibration]
 
 Headline workload: 1M Gaussians (means N(0, 2 I), log-scales N(ln .01, .5),
@@ -24,6 +24,11 @@ Legs beside the headline (each in its own key of the line):
            3840x2160} and 20% forward+backward: saturation throughput of the
            continuously batching server against one-at-a-time plain renders,
            and p50/p99 latency under Poisson arrivals at a stated rate.
+  optim    on the headline scene: an Adam step (pos and opacity, as the
+           reference; and all 59 parameters), densify_and_prune with the Adam
+           moments carried, each against the HBM roofline with its
+           algorithmic bytes; and whole training steps (forward, loss,
+           backward, Adam) per second.
 
 Throughput is timed over back-to-back frames; per-stage GPU time comes from
 HIP events the library records on the render stream around each stage, in a
@@ -414,6 +419,92 @@ def serve_leg(args):
     return res
 
 
+def optim_leg(args):
+    """Adam (reference planes and all 59), densify_and_prune with the Adam
+    moments, and one whole training step (forward, loss, backward, Adam) on
+    the headline's 1M-Gaussian scene."""
+    import numpy as np
+    import torch
+    from lumen_amd.optim import (AdamOptimizer, TrainingConfig, TrainingCoordinator,
+                                 densify_and_prune)
+    from lumen_amd.raster import Rasterizer, RenderConfig
+    from lumen_amd.scene import perturbed, workload
+    from lumen_amd.training import GaussianGradients
+    scene, cam = workload(args.workload)
+    n = scene.count
+    rng = np.random.default_rng(5)
+    cfg = TrainingConfig()
+    res = {"workload": args.workload, "gaussians": n}
+
+    def events_ms(fn, steps):
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        tot = 0.0
+        for _ in range(steps):
+            s.record()
+            fn()
+            e.record()
+            e.synchronize()
+            tot += s.elapsed_time(e)
+        return tot / steps
+
+    dev = scene.to_device("cuda:0")
+    g = GaussianGradients(n)
+    g.planes.copy_(torch.from_numpy(rng.normal(0, 1e-3, (11, n)).astype(np.float32)))
+    g.sh.copy_(torch.from_numpy(rng.normal(0, 1e-3, (n, 48)).astype(np.float32)))
+    for all_params, key, per in ((False, "adam", 112), (True, "adam_all_params", 59 * 28)):
+        opt = AdamOptimizer(n, all_params=all_params)
+        for _ in range(args.warmup):
+            opt.step(dev, g, cfg)
+        ms = events_ms(lambda: opt.step(dev, g, cfg), args.steps)
+        res[key] = roofline_entry("hbm", per * n, ms)
+        del opt
+    # densify: a mix of prunes, clones and splits, with the moments carried
+    g.viewspace_grad_2d.copy_(torch.from_numpy(rng.exponential(0.0003, n).astype(np.float32)))
+    g.observation_count.copy_(torch.from_numpy(rng.integers(0, 4, n).astype(np.int32)))
+    opt = AdamOptimizer(n, all_params=True)
+    R = Rasterizer(0)
+    new, st = densify_and_prune(dev, g, cfg, 4.0, None, R)   # sizes the scratch
+    m = new.count
+    del new
+
+    def dens():
+        o = AdamOptimizer.__new__(AdamOptimizer)
+        o.m_planes, o.m_sh, o.v_planes, o.v_sh = opt.m_planes, opt.m_sh, opt.v_planes, opt.v_sh
+        densify_and_prune(dev, g, cfg, 4.0, o, R)
+    dens()
+    ms = events_ms(dens, max(3, args.steps // 5))
+    # classify 29 B + scan 12 B + emit 5 B per source; 236 B read and written
+    # per output Gaussian, and its two moments (2 x 236 B each way)
+    work = n * (29 + 12 + 5) + m * (236 * 2 + 944 * 2)
+    res["densify_and_prune"] = dict(roofline_entry("hbm", work, ms), cloned=st.cloned,
+                                    split=st.split, pruned=st.pruned, output=m,
+                                    note="includes the host read-back of the new count and the "
+                                         "compaction copies of the outputs")
+    del opt, g
+    torch.cuda.empty_cache()
+    # one training step of the reference's train_step, with the real backward,
+    # on a fresh copy of the scene (the Adam timing above moved its planes)
+    del dev
+    dev = scene.to_device("cuda:0")
+    target = R.render(perturbed(scene, 0.01, 7).to_device("cuda:0"), cam, RenderConfig()).rgb.clone()
+    coord = TrainingCoordinator(TrainingConfig(densify_from_iter=10 ** 9), RenderConfig(), R)
+    opt = AdamOptimizer(n)
+    it = [0]
+
+    def step():
+        it[0] += 1
+        nonlocal dev
+        dev, _ = coord.train_step(dev, cam, target, opt, it[0], 4.0)
+    for _ in range(args.warmup):
+        step()
+    ms_step = timed(step, args.steps) * 1e3
+    res["train_step"] = {"value": round(1e3 / ms_step, 2), "unit": "train steps/s",
+                         "ms_per_step": round(ms_step, 4),
+                         "contents": "forward, L1 + D-SSIM loss, backward, Adam (pos, opacity)"}
+    R.close()
+    return res
+
+
 def copy_calibration():
     """A plain device copy of known size (for scaling rocprofv3's FETCH_SIZE /
     WRITE_SIZE counters): 256 MiB float32, 5 times."""
@@ -432,8 +523,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="random1m_1920x1080")
-    ap.add_argument("--legs", default="fwd_bwd,batch,serve",
-                    help="comma list of fwd_bwd, batch, serve ('' for the headline only)")
+    ap.add_argument("--legs", default="fwd_bwd,batch,serve,optim",
+                    help="comma list of fwd_bwd, batch, serve, optim ('' for the headline only)")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-backward", action="store_true", help="same as leaving fwd_bwd out")
@@ -457,7 +548,8 @@ def main() -> int:
 
     result = {}
     scene, cam, cfg, V, M = headline(args, result)
-    for leg, key, fn in (("batch", "batched", batch_leg), ("serve", "serving", serve_leg)):
+    for leg, key, fn in (("batch", "batched", batch_leg), ("serve", "serving", serve_leg),
+                         ("optim", "optimizer", optim_leg)):
         if leg in args.legs:
             try:
                 result[key] = fn(args)
