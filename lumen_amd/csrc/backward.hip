@@ -98,14 +98,16 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   __syncthreads();
   int last[PIX];
   float T[PIX], gC0[PIX], gC1[PIX], gC2[PIX], gD[PIX], Tf[PIX], gsum[PIX];
-  float SC0[PIX], SC1[PIX], SC2[PIX], SD[PIX], pyc[PIX];
+  // Q = sum over channels of dL/dC_k times the colour (and depth) blended
+  // behind the current splat, which is all the opacity gradient needs of it
+  float Q[PIX], pyc[PIX];
   int lane_last = rg.x - 1;
 #pragma unroll
   for (int p = 0; p < PIX; ++p) {
     last[p] = rg.x - 1;
     T[p] = 1.0f;
     gC0[p] = gC1[p] = gC2[p] = gD[p] = 0.0f;
-    SC0[p] = SC1[p] = SC2[p] = SD[p] = 0.0f;
+    Q[p] = 0.0f;
     pyc[p] = (float)(py0 + p) + 0.5f;
     if (px < v.width && py0 + p < v.height) {
       const size_t pix = v.pix_base + (size_t)(py0 + p) * v.width + px;
@@ -200,13 +202,12 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
               const float rom = __builtin_amdgcn_rcpf(om);
               const float Tj = T[p] * rom;
               const float w = alpha * Tj;
-              float dal = gC0[p] * (c.x * Tj - SC0[p] * rom) + gC1[p] * (c.y * Tj - SC1[p] * rom) +
-                          gC2[p] * (c.z * Tj - SC2[p] * rom) + gD[p] * (c.w * Tj - SD[p] * rom);
+              // dL/dalpha = sum_k g_k (c_k Tj - S_k / (1 - alpha)), S_k the
+              // channel blended behind; sum_k g_k S_k is carried as Q
+              const float gc = gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z + gD[p] * c.w;
+              float dal = gc * Tj - Q[p] * rom;
               if (white) dal -= gsum[p] * Tf[p] * rom;
-              SC0[p] += c.x * w;
-              SC1[p] += c.y * w;
-              SC2[p] += c.z * w;
-              SD[p] += c.w * w;
+              Q[p] += gc * w;
               T[p] = Tj;
               wm = w;
               if (!aclamp) {
