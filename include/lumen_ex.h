@@ -75,11 +75,15 @@ typedef struct {
     int32_t compute_depth;
     uint32_t flags;          /* LUMEN_EX_* below */
     uint32_t batch_depth;    /* compositor LDS batch: 64, 128 or 256; 0 = 128 for tiles of a
-                                multiple of 128 pixels (two pixels per lane), else 256 */
+                                multiple of 128 pixels (two pixels per lane), else 256.
+                                The backward's batch is 128 for 128 or 256, else 64 */
 } LumenExRenderConfig;
 
 #define LUMEN_EX_KEEP_STAGES 1u   /* keep the unsorted keys/values for lumen_ex_copy_stage */
 #define LUMEN_EX_SAVE_PIXELS 2u   /* keep final transmittance and evaluation counts per pixel */
+#define LUMEN_EX_GRAD_ASSIGN 8u   /* backward: write the gradient planes (every Gaussian's entry,
+                                     zero where it is not visible) instead of adding to them;
+                                     viewspace_grad_2d / observation_count still accumulate */
 
 typedef struct {
     uint64_t visible_splats;        /* RenderOutput.visible_splats (pipeline.mojo:159) */
@@ -197,7 +201,8 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* ctx, int stage, void* dst_host, 
 
 /* Gradient planes in the layout of GaussianGradients
  * (src/optim/gradients.mojo:10-106), device pointers, ACCUMULATED into (the
- * caller zeroes them, as GaussianGradients.zero does).  viewspace_grad_2d and
+ * caller zeroes them, as GaussianGradients.zero does), or written when the
+ * backward's config has LUMEN_EX_GRAD_ASSIGN (then they need no zeroing).  viewspace_grad_2d and
  * observation_count follow accumulate_viewspace_grad (gradients.mojo:96-99):
  * every Gaussian visible in the view gets |dL/d mean2d| (pixels) added and
  * its count incremented.  Either of the two may be NULL. */
@@ -221,7 +226,8 @@ typedef struct {
 /* Backward of the last lumen_ex_render on this context, which must have
  * been called with LUMEN_EX_SAVE_PIXELS and the same scene, camera and config.
  * grad_rgb: H*W*3 floats, grad_depth: H*W floats or NULL (device).  Gradients
- * are ADDED to the planes of grads.  The compositor's reverse pass recomputes
+ * are ADDED to the planes of grads (written with LUMEN_EX_GRAD_ASSIGN in
+ * config->flags).  The compositor's reverse pass recomputes
  * every alpha exactly as the forward did, recovers transmittance from the
  * saved final value, and stores each (splat, tile) instance's partial
  * gradient in its own slot; a second kernel sums a splat's slots in a fixed

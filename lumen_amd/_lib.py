@@ -100,6 +100,7 @@ class LumenExGradients(ctypes.Structure):
 LUMEN_EX_KEEP_STAGES = 1
 LUMEN_EX_SAVE_PIXELS = 2
 LUMEN_EX_GRAPH = 4
+LUMEN_EX_GRAD_ASSIGN = 8
 LUMEN_EX_ERROR_CAPACITY = -101
 
 STAGES = {"vis_to_orig": 0, "records": 1, "radius": 2, "keys_unsorted": 3, "vals_unsorted": 4,

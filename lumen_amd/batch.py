@@ -131,12 +131,15 @@ class MultiViewRenderer:
         self.stream.synchronize()
 
     def backward(self, scene: DeviceScene, cameras: Sequence[PinholeCamera], config: RenderConfig,
-                 grad_rgb: torch.Tensor, grad_depth: Optional[torch.Tensor], grads) -> None:
+                 grad_rgb: torch.Tensor, grad_depth: Optional[torch.Tensor], grads,
+                 assign: bool = False) -> None:
+        """Accumulate (or with assign=True write) the gradient of the last
+        frame into `grads` (a GaussianGradients or a LumenExGradients)."""
         n = len(cameras)
         cams = (_lib.LumenExCamera * n)(*[camera_struct(c) for c in cameras])
-        cfg = config.ex_struct(0)
+        cfg = config.ex_struct(_lib.LUMEN_EX_GRAD_ASSIGN if assign else 0)
         sc = scene.ex_struct()
-        g = grads.ex_struct()
+        g = grads if isinstance(grads, _lib.LumenExGradients) else grads.ex_struct()
         self.stream.wait_stream(torch.cuda.current_stream(scene.device))
         st = _lib.lib().lumen_ex_render_batch_backward(
             self.rasterizer.handle, ctypes.byref(sc), cams, n, ctypes.byref(cfg),
@@ -178,20 +181,17 @@ class RasterizeViews(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_rgb, grad_depth):
-        from .training import GaussianGradients
+        from .training import new_gradients
         R = ctx.renderer
         out_rgb, out_depth = ctx.saved_tensors
         if R.rasterizer.frame_id != ctx.frame:
             R.render(ctx.scene, ctx.cameras, ctx.config, graph=ctx.graph, save_pixels=True,
                      out=(out_rgb, out_depth))
-        grads = GaussianGradients(ctx.scene.count, ctx.scene.device)
-        if ctx.viewspace is not None:
-            grads.viewspace_grad_2d = ctx.viewspace.viewspace_grad_2d
-            grads.observation_count = ctx.viewspace.observation_count
+        planes, sh, gs = new_gradients(ctx.scene.count, ctx.scene.device, ctx.viewspace)
         g_rgb = grad_rgb.contiguous() if grad_rgb is not None else torch.zeros_like(out_rgb)
         g_d = grad_depth.contiguous() if grad_depth is not None else None
-        R.backward(ctx.scene, ctx.cameras, ctx.config, g_rgb, g_d, grads)
-        return grads.planes, grads.sh, None, None, None, None, None
+        R.backward(ctx.scene, ctx.cameras, ctx.config, g_rgb, g_d, gs, assign=True)
+        return planes, sh, None, None, None, None, None
 
 
 def rasterize_views(planes, sh, cameras, config=None, renderer=None, graph=False, viewspace=None):

@@ -20,9 +20,9 @@
 //                     stored (no atomics) in the slot of that (splat, tile) instance in emission
 //                     order: the splat's first key offset plus the tile's
 //                     position in its rectangle, row-major.
-//   k_sum_partial     sums each visible splat's instance slots in a fixed
-//                     order (so the result does not depend on scheduling and
-//                     is the same run to run).
+//   k_sum_chunks      sums each visible splat's instance slots in a fixed
+//   + k_sum_fixup     order (so the result does not depend on scheduling and
+//                     is the same run to run), chunk by chunk through LDS.
 //   k_preprocess_bwd  one thread per Gaussian, over every view it is visible
 //                     in: adjoints of the SH colour (and view direction), the
 //                     projection to screen, the 2D inverse and the EWA
@@ -52,11 +52,23 @@ __device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ vie
   return lo;
 }
 
-__global__ void k_zero_partial(float* __restrict__ g, const SortPlan* __restrict__ plan) {
+// Zeroes the frame's instance slots (a slot the reverse pass never reaches
+// stays zero) and records, for every kSumChunk-slot chunk, the visible splat
+// whose slots contain the chunk's first slot.
+__global__ void k_zero_partial(float* __restrict__ g, const SortPlan* __restrict__ plan,
+                               const uint4* __restrict__ vmeta,
+                               uint32_t* __restrict__ chunk_seg) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t n = (uint64_t)plan->m * kGradVis;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    g[i] = 0.0f;
+  for (uint64_t i = i0; i < n; i += stride) g[i] = 0.0f;
+  const uint32_t V = plan->v, M = plan->m;
+  for (uint64_t k = i0; k < V; k += stride) {
+    const uint32_t q0 = vmeta[k].x;
+    const uint32_t q1 = k + 1u < V ? vmeta[k + 1u].x : M;
+    for (uint32_t c = (q0 + kSumChunk - 1u) / kSumChunk; c * kSumChunk < q1; ++c)
+      chunk_seg[c] = (uint32_t)k;
+  }
 }
 
 template <int BATCH, int PIX>
@@ -251,72 +263,133 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   }
 }
 
-// Sum of each visible splat's tile-instance partials (its emission-order
-// slots [first key, next splat's first key)), in a fixed order so the result
-// does not depend on scheduling: a lane sums a splat of up to kSumLane
-// instances sequentially; the wave takes larger splats one at a time, lane l
-// summing instances l, l + 64, ... and the wave folding the 64 lane sums with
-// DPP.  One thread per visible splat.
+// The per-splat sums of the instance slots, in a fixed order (so the result
+// does not depend on scheduling).  A splat's slots are contiguous and the
+// splats' runs follow each other in visible order, so the slots are cut into
+// chunks of kSumChunk: one block per chunk loads it into LDS with coalesced
+// reads, then one thread per splat meeting the chunk sums its slots there in
+// order.  A splat inside the chunk is written out; the piece of a splat that
+// continues past the chunk ("open") and the piece of one that began before it
+// ("cont", also a whole chunk inside one splat) go to aux, and k_sum_fixup
+// adds a crossing splat's pieces in chunk order.
+// Portions of at most kSumLane slots are summed by one thread; longer ones
+// (at most kSumChunk / (kSumLane + 1) per chunk) by a wave, lane l taking
+// slots l, l + 64, ... and the wave folding its 64 sums with DPP.
 constexpr uint32_t kSumLane = 32;
 
-__global__ __launch_bounds__(256) void k_sum_partial(const uint4* __restrict__ vmeta,
-                                                     const float* __restrict__ partial,
-                                                     const SortPlan* __restrict__ plan,
-                                                     float* __restrict__ grad_vis) {
+__device__ __forceinline__ void sum_store(const float (&g)[kGradVis], float* dst) {
+  float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+  for (int c = 0; c < kGradVis / 2; ++c) d2[c] = make_float2(g[2 * c], g[2 * c + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_sum_chunks(const uint4* __restrict__ vmeta,
+                                                    const float* __restrict__ partial,
+                                                    const SortPlan* __restrict__ plan,
+                                                    const uint32_t* __restrict__ chunk_seg,
+                                                    float* __restrict__ grad_vis,
+                                                    float* __restrict__ aux) {
+  __shared__ float2 s_rec[kSumChunk * kGradVis / 2];
+  __shared__ uint32_t s_long[kSumChunk / (kSumLane + 1) + 1];
+  __shared__ uint32_t s_nlong;
   if (plan->overflow) return;
   const uint32_t V = plan->v, M = plan->m;
-  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  uint32_t q0 = 0, q1 = 0;
-  if (k < V) {
-    q0 = vmeta[k].x;
-    q1 = k + 1u < V ? vmeta[k + 1u].x : M;
-  }
-  const bool big = q1 - q0 > kSumLane;
-  float g[kGradVis];
+  const uint32_t c0 = blockIdx.x * kSumChunk;
+  if (c0 >= M || V == 0) return;  // uniform across the block
+  const uint32_t c1 = min(c0 + kSumChunk, M);
+  if (threadIdx.x == 0) s_nlong = 0;
+  const float2* src = reinterpret_cast<const float2*>(partial + (size_t)c0 * kGradVis);
+  const uint32_t n2 = (c1 - c0) * (kGradVis / 2);
+  for (uint32_t e = threadIdx.x; e < n2; e += 256u) s_rec[e] = src[e];
+  __syncthreads();
+  // where a splat's portion of this chunk goes: the splat's sum when the
+  // splat lies inside the chunk, else aux ("cont" if it began before, "open")
+  auto dst_of = [&](uint32_t k, uint32_t q0, uint32_t q1) -> float* {
+    return q0 >= c0 && q1 <= c1 ? grad_vis + (size_t)k * kGradVis
+           : q0 < c0            ? aux + ((size_t)blockIdx.x * 2u + 1u) * kGradVis
+                                : aux + (size_t)blockIdx.x * 2u * kGradVis;
+  };
+  const uint32_t seg0 = chunk_seg[blockIdx.x];
+  for (uint32_t k = seg0 + threadIdx.x; k < V; k += 256u) {
+    const uint32_t q0 = vmeta[k].x;
+    if (q0 >= c1) break;
+    const uint32_t q1 = k + 1u < V ? vmeta[k + 1u].x : M;
+    const uint32_t a = max(q0, c0), b = min(q1, c1);
+    if (b > a + kSumLane) {  // a wave sums it below
+      s_long[atomicAdd(&s_nlong, 1u)] = k;
+      continue;
+    }
+    float g[kGradVis];
 #pragma unroll
-  for (int c = 0; c < kGradVis; ++c) g[c] = 0.0f;
-  if (!big) {
-    for (uint32_t q = q0; q < q1; ++q) {
-      const float2* pp = reinterpret_cast<const float2*>(partial + (size_t)q * kGradVis);
+    for (int c = 0; c < kGradVis; ++c) g[c] = 0.0f;
+    for (uint32_t q = a; q < b; ++q) {
+      const float2* r = &s_rec[(q - c0) * (kGradVis / 2)];
 #pragma unroll
       for (int c = 0; c < kGradVis / 2; ++c) {
-        const float2 x = pp[c];
+        const float2 x = r[c];
         g[2 * c] += x.x;
         g[2 * c + 1] += x.y;
       }
     }
+    sum_store(g, dst_of(k, q0, q1));
   }
-  uint64_t todo = __ballot(big);
-  while (todo) {  // wave-uniform
-    const int src = __builtin_ctzll(todo);
-    todo &= todo - 1ull;
-    const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)q0, src);
-    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)q1, src);
-    float h[kGradVis];
+  __syncthreads();
+  const uint32_t nlong = s_nlong;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = threadIdx.x >> 6; i < nlong; i += 4u) {  // wave-uniform
+    const uint32_t k = s_long[i];
+    const uint32_t q0 = vmeta[k].x;
+    const uint32_t q1 = k + 1u < V ? vmeta[k + 1u].x : M;
+    const uint32_t a = max(q0, c0), b = min(q1, c1);
+    float g[kGradVis];
 #pragma unroll
-    for (int c = 0; c < kGradVis; ++c) h[c] = 0.0f;
-    for (uint32_t q = b0 + (uint32_t)lane; q < b1; q += 64u) {
-      const float2* pp = reinterpret_cast<const float2*>(partial + (size_t)q * kGradVis);
+    for (int c = 0; c < kGradVis; ++c) g[c] = 0.0f;
+    for (uint32_t q = a + (uint32_t)lane; q < b; q += 64u) {
+      const float2* r = &s_rec[(q - c0) * (kGradVis / 2)];
 #pragma unroll
       for (int c = 0; c < kGradVis / 2; ++c) {
-        const float2 x = pp[c];
-        h[2 * c] += x.x;
-        h[2 * c + 1] += x.y;
+        const float2 x = r[c];
+        g[2 * c] += x.x;
+        g[2 * c + 1] += x.y;
       }
     }
 #pragma unroll
-    for (int c = 0; c < kGradVis; ++c) {
-      const float tot = wave_sum_lane63(h[c]);
-      const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tot), 63));
-      if (lane == src) g[c] = b;
-    }
+    for (int c = 0; c < kGradVis; ++c)
+      g[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wave_sum_lane63(g[c])), 63));
+    if (lane == 0) sum_store(g, dst_of(k, q0, q1));
   }
-  if (k < V) {
-    float2* out = reinterpret_cast<float2*>(grad_vis + (size_t)k * kGradVis);
+}
+
+// Splats whose slots cross a chunk boundary: open piece of the first chunk,
+// then the cont pieces of the following ones, in order.
+__global__ __launch_bounds__(256) void k_sum_fixup(const uint4* __restrict__ vmeta,
+                                                   const SortPlan* __restrict__ plan,
+                                                   const float* __restrict__ aux,
+                                                   float* __restrict__ grad_vis) {
+  if (plan->overflow) return;
+  const uint32_t V = plan->v, M = plan->m;
+  const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+  if (k >= V) return;
+  const uint32_t q0 = vmeta[k].x;
+  const uint32_t q1 = k + 1u < V ? vmeta[k + 1u].x : M;
+  if (q1 <= q0) {  // no instance slot (a chunk may not reach such a splat)
 #pragma unroll
-    for (int c = 0; c < kGradVis / 2; ++c) out[c] = make_float2(g[2 * c], g[2 * c + 1]);
+    for (int c = 0; c < kGradVis; ++c) grad_vis[(size_t)k * kGradVis + c] = 0.0f;
+    return;
   }
+  const uint32_t ca = q0 / kSumChunk, cb = (q1 - 1u) / kSumChunk;
+  if (ca == cb) return;
+  float g[kGradVis];
+  const float* o = aux + (size_t)ca * 2u * kGradVis;
+#pragma unroll
+  for (int c = 0; c < kGradVis; ++c) g[c] = o[c];
+  for (uint32_t ch = ca + 1u; ch <= cb; ++ch) {
+    const float* t = aux + ((size_t)ch * 2u + 1u) * kGradVis;
+#pragma unroll
+    for (int c = 0; c < kGradVis; ++c) g[c] += t[c];
+  }
+#pragma unroll
+  for (int c = 0; c < kGradVis; ++c) grad_vis[(size_t)k * kGradVis + c] = g[c];
 }
 
 // dY_k/d(x, y, z) of the real SH polynomials of sh_basis, contracted with
@@ -392,9 +465,14 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const uint64_t* __restrict__ totals,
                                                         const SortPlan* __restrict__ plan,
                                                         const float* __restrict__ grad_vis,
-                                                        GradPlanes out) {
+                                                        GradPlanes out, bool assign) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= s.n || plan->overflow) return;
+  // the gradient planes are added to, or written (assign)
+  auto put = [&](int plane, float g) {
+    if (assign) out.p[plane][i] = g;
+    else out.p[plane][i] += g;
+  };
   const uint64_t nv = (uint64_t)nviews * s.n;
   const float px = s.px[i], py = s.py[i], pz = s.pz[i];
   const int ncoef = views[0].sh_coeffs;  // the same in every view of a frame
@@ -552,19 +630,32 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
     gpc[1] += gj12 * (-fy * itz2);
     for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
   }
-  if (!seen) return;
+  float4* gsh4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
+  if (!seen) {  // no gradient; written as zero when assigning
+    if (assign) {
+      for (int k = 0; k < 11; ++k) out.p[k][i] = 0.0f;
+#pragma unroll
+      for (int qd = 0; qd < 12; ++qd) gsh4[qd] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return;
+  }
 
-  out.p[10][i] += gop;
-  out.p[0][i] += gpos[0];
-  out.p[1][i] += gpos[1];
-  out.p[2][i] += gpos[2];
+  put(10, gop);
+  put(0, gpos[0]);
+  put(1, gpos[1]);
+  put(2, gpos[2]);
   if (out.viewspace) out.viewspace[i] += vs;
   if (out.count) out.count[i] += seen;
-  // dL/dsh as float4s (elements past the used coefficients are left as they are)
-  float4* gsh4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
+  // dL/dsh as float4s (assigning: every element, zero past the used
+  // coefficients; adding: those elements are left as they are)
+  if (assign) {
+#pragma unroll
+    for (int qd = 0; qd < 12; ++qd)
+      gsh4[qd] = make_float4(gsh[4 * qd], gsh[4 * qd + 1], gsh[4 * qd + 2], gsh[4 * qd + 3]);
+  }
 #pragma unroll
   for (int qd = 0; qd < 12; ++qd) {
-    if (4 * qd >= nfloat) continue;
+    if (assign || 4 * qd >= nfloat) continue;
     float4 o = gsh4[qd];
     if (4 * qd + 0 < nfloat) o.x += gsh[4 * qd + 0];
     if (4 * qd + 1 < nfloat) o.y += gsh[4 * qd + 1];
@@ -580,7 +671,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
       gM[3 * rr + cc] = 2.0f * (F[rr][0] * M[cc] + F[rr][1] * M[3 + cc] + F[rr][2] * M[6 + cc]);
   for (int cc = 0; cc < 3; ++cc) {
     const float gs = gM[cc] * R[cc] + gM[3 + cc] * R[3 + cc] + gM[6 + cc] * R[6 + cc];
-    if (!sclamp[cc]) out.p[3 + cc][i] += gs;
+    put(3 + cc, sclamp[cc] ? 0.0f : gs);  // the 1e-5 clamp passes no gradient
   }
   for (int rr = 0; rr < 3; ++rr)
     for (int cc = 0; cc < 3; ++cc) gR[3 * rr + cc] = gM[3 * rr + cc] * sc[cc];
@@ -596,7 +687,9 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
   if (qn > 0.000001f) {
     const float dot = gq[0] * w + gq[1] * qx + gq[2] * qy + gq[3] * qz;
     const float qq[4] = {w, qx, qy, qz};
-    for (int a = 0; a < 4; ++a) out.p[6 + a][i] += (gq[a] - qq[a] * dot) / qn;
+    for (int a = 0; a < 4; ++a) put(6 + a, (gq[a] - qq[a] * dot) / qn);
+  } else if (assign) {  // the identity fallback passes no gradient
+    for (int a = 0; a < 4; ++a) out.p[6 + a][i] = 0.0f;
   }
 }
 
@@ -638,43 +731,41 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   const uint64_t m_cap = c->sizes_known ? c->M : c->m_cap;
   LUMEN_TRY(ensure(c->partial, partial_bytes(m_cap)));
   LUMEN_TRY(ensure(c->grad_vis, grad_vis_bytes(v_cap)));
+  const uint32_t nchunks = div_up(m_cap, kSumChunk);
+  LUMEN_TRY(ensure(c->sum_seg, sum_seg_bytes(m_cap)));
+  LUMEN_TRY(ensure(c->sum_aux, sum_aux_bytes(m_cap)));
   float* gv = c->grad_vis.as<float>();
   const SortPlan* plan = c->plan.as<SortPlan>();
   float* part = c->partial.as<float>();
+  const uint4* vmeta = c->vmeta.as<uint4>();
   const uint32_t zb = div_up(m_cap * kGradVis, 1024) < 8192 ? div_up(m_cap * kGradVis, 1024) : 8192;
-  hipLaunchKernelGGL(k_zero_partial, dim3(zb ? zb : 1), dim3(256), 0, st, part, plan);
+  hipLaunchKernelGGL(k_zero_partial, dim3(zb ? zb : 1), dim3(256), 0, st, part, plan, vmeta,
+                     c->sum_seg.as<uint32_t>());
   LUMEN_LAUNCHED();
   const uint32_t threads = cfg->tile_width * cfg->tile_height;
-  // at most 128 splats per LDS batch: the per-wave gradient slots of 256
-  // would cap occupancy at 3 waves per SIMD
-  const uint32_t batch = cfg->batch_depth == 64u ? 64u : 128u;
+  // 64 splats per LDS batch by default: the per-wave gradient slots (40 B
+  // per splat per wave) then leave room for 8 waves per SIMD, which measured
+  // 6% faster than 128 (5 waves); 128 when the config asks for 128 or 256
+  const uint32_t batch = (cfg->batch_depth == 128u || cfg->batch_depth == 256u) ? 128u : 64u;
   const int2* ranges = c->ranges.as<int2>();
   const uint32_t* v0 = c->vals[0].as<uint32_t>();
   const uint32_t* v1 = c->vals[1].as<uint32_t>();
   const float4* rec = c->rec.as<float4>();
   const ViewParams* views = c->views.as<ViewParams>();
   const ViewSpan* spans = c->spans.as<ViewSpan>();
-  const uint4* vmeta = c->vmeta.as<uint4>();
   const float* tf = c->t_final.as<float>();
   const int32_t* ne = c->n_eval.as<int32_t>();
   const int nv = (int)c->nviews;
-  // one pixel per lane: the reverse pass keeps four sums and the adjoint
-  // terms per pixel, and two pixels per lane measured no faster (fewer waves
-  // resident)
-  const bool two = false;
-  const dim3 grid((uint32_t)c->T), block(two ? threads / 2 : threads);
+  // one pixel per lane: two pixels per lane measured 4% slower (the adjoint
+  // state of two pixels costs resident waves)
+  const dim3 grid((uint32_t)c->T), block(threads);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
-#define LUMEN_COMPOSITE_BWD(B, P)                                                             \
-  hipLaunchKernelGGL((k_composite_bwd<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec,  \
-                     views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part)
-  if (two) {
-    if (batch == 64) LUMEN_COMPOSITE_BWD(64, 2);
-    else LUMEN_COMPOSITE_BWD(128, 2);
-  } else {
-    if (batch == 64) LUMEN_COMPOSITE_BWD(64, 1);
-    else LUMEN_COMPOSITE_BWD(128, 1);
-  }
-#undef LUMEN_COMPOSITE_BWD
+  if (batch == 64)
+    hipLaunchKernelGGL((k_composite_bwd<64, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
+                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
+  else
+    hipLaunchKernelGGL((k_composite_bwd<128, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
+                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, true, st));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, false, st));
@@ -683,12 +774,17 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   gp.sh = grads->sh;
   gp.viewspace = grads->viewspace_grad_2d;
   gp.count = grads->observation_count;
-  hipLaunchKernelGGL(k_sum_partial, dim3(div_up(v_cap, 256)), dim3(256), 0, st, vmeta, part, plan,
+  float* aux = c->sum_aux.as<float>();
+  hipLaunchKernelGGL(k_sum_chunks, dim3(nchunks ? nchunks : 1), dim3(256), 0, st, vmeta, part,
+                     plan, c->sum_seg.as<uint32_t>(), gv, aux);
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_sum_fixup, dim3(div_up(v_cap, 256)), dim3(256), 0, st, vmeta, plan, aux,
                      gv);
   LUMEN_LAUNCHED();
   hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(sc->count, 256)), dim3(256), 0, st,
                      scene_dev(sc), views, nv, c->packed.as<uint64_t>(),
-                     c->totals.as<uint64_t>(), plan, gv, gp);
+                     c->totals.as<uint64_t>(), plan, gv, gp,
+                     (cfg->flags & LUMEN_EX_GRAD_ASSIGN) != 0);
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, true, st));
   return LUMEN_SUCCESS;

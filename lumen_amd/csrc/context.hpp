@@ -58,6 +58,13 @@ inline SceneDev scene_dev(const LumenExScene* sc) {
 constexpr int kGradVis = 10;
 inline size_t partial_bytes(uint64_t m_cap) { return (m_cap ? m_cap : 1) * kGradVis * sizeof(float); }
 inline size_t grad_vis_bytes(uint64_t v_cap) { return (v_cap ? v_cap : 1) * kGradVis * sizeof(float); }
+// The per-splat sum cuts the instance slots into chunks of kSumChunk: per
+// chunk, the splat holding its first slot and two partial sums.
+constexpr uint32_t kSumChunk = 512;
+inline size_t sum_seg_bytes(uint64_t m_cap) { return (size_t)div_up(m_cap ? m_cap : 1, kSumChunk) * 4; }
+inline size_t sum_aux_bytes(uint64_t m_cap) {
+  return (size_t)div_up(m_cap ? m_cap : 1, kSumChunk) * 2 * kGradVis * sizeof(float);
+}
 
 // lumen_ex_photometric_loss's partial sums for a width x height image: two
 // floats per block of four 8x8 windows and per block of the pixels outside
@@ -103,6 +110,7 @@ struct LumenExContext {
   lumen::DevBuf ranges, t_final, n_eval, minmax;
   lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
+  lumen::DevBuf sum_seg, sum_aux;  // backward: per slot chunk, its first splat and partial sums
   lumen::DevBuf loss_part; // lumen_ex_photometric_loss: per-block partial sums
   // lumen_ex_densify_and_prune: per-Gaussian class, output count and offset;
   // stats, total and the scan's partial sums

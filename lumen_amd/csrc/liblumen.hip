@@ -1059,8 +1059,8 @@ static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
           &c->geo,    &c->block_bounds, &c->rec,   &c->vis2orig,  &c->radius,   &c->vmeta,
           &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
           &c->hist,   &c->hist_part, &c->ranges,   &c->t_final,   &c->n_eval,   &c->minmax,
-          &c->partial, &c->grad_vis, &c->loss_part, &c->dens_cls, &c->dens_cnt, &c->dens_off,
-          &c->dens_misc};
+          &c->partial, &c->grad_vis, &c->sum_seg, &c->sum_aux, &c->loss_part, &c->dens_cls,
+          &c->dens_cnt, &c->dens_off, &c->dens_misc};
 }
 
 struct WorkspaceShape {
@@ -1115,6 +1115,8 @@ static LumenStatus workspace(LumenExContext* c, Need&& need, uint64_t N, const W
   if (backward) {
     LUMEN_TRY(need(c->partial, partial_bytes(m_cap)));
     LUMEN_TRY(need(c->grad_vis, grad_vis_bytes(v_cap)));
+    LUMEN_TRY(need(c->sum_seg, sum_seg_bytes(m_cap)));
+    LUMEN_TRY(need(c->sum_aux, sum_aux_bytes(m_cap)));
     LUMEN_TRY(need(c->loss_part, loss_part_bytes(ws.max_w, ws.max_h)));
   }
   return LUMEN_SUCCESS;

@@ -166,14 +166,15 @@ class Rasterizer:
         return out
 
     def render_backward(self, scene, camera: PinholeCamera, config: RenderConfig, grad_rgb,
-                        grad_depth, grads: "GaussianGradients") -> None:
+                        grad_depth, grads, assign: bool = False) -> None:
         """Accumulate the gradient of the last render (made with save_pixels=True
-        from the same scene/camera/config) into `grads`."""
+        from the same scene/camera/config) into `grads` (a GaussianGradients or
+        a LumenExGradients struct), or write it there with assign=True."""
         assert grad_rgb.is_contiguous() and grad_rgb.dtype.is_floating_point
         sc = scene.ex_struct()
         cam = camera_struct(camera)
-        cfg = config.ex_struct(0)
-        g = grads.ex_struct()
+        cfg = config.ex_struct(_lib.LUMEN_EX_GRAD_ASSIGN if assign else 0)
+        g = grads if isinstance(grads, _lib.LumenExGradients) else grads.ex_struct()
         gd = grad_depth.data_ptr() if grad_depth is not None else None
         st = _lib.lib().lumen_ex_render_backward(self._ctx, ctypes.byref(sc), ctypes.byref(cam),
                                                  ctypes.byref(cfg), grad_rgb.data_ptr(), gd,

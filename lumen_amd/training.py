@@ -64,16 +64,34 @@ class GaussianGradients:
         return torch.where(c > 0, self.viewspace_grad_2d / c.clamp_min(1.0), torch.zeros_like(c))
 
     def ex_struct(self):
-        from ._lib import LumenExGradients
-        g = LumenExGradients()
-        n = self.count
-        base = self.planes.data_ptr()
-        for k, name in enumerate(PLANE_NAMES):
-            setattr(g, name, base + 4 * k * n)
-        g.sh = self.sh.data_ptr()
-        g.viewspace_grad_2d = self.viewspace_grad_2d.data_ptr()
-        g.observation_count = self.observation_count.data_ptr()
-        return g
+        return gradients_struct(self.planes, self.sh, self.viewspace_grad_2d,
+                                self.observation_count)
+
+
+def gradients_struct(planes, sh, viewspace_grad_2d=None, observation_count=None):
+    """LumenExGradients over (11, N) gradient planes and (N, 48) SH gradients;
+    the view-space accumulators may be left out (NULL)."""
+    from ._lib import LumenExGradients
+    g = LumenExGradients()
+    n = planes.shape[1]
+    base = planes.data_ptr()
+    for k, name in enumerate(PLANE_NAMES):
+        setattr(g, name, base + 4 * k * n)
+    g.sh = sh.data_ptr()
+    g.viewspace_grad_2d = viewspace_grad_2d.data_ptr() if viewspace_grad_2d is not None else None
+    g.observation_count = observation_count.data_ptr() if observation_count is not None else None
+    return g
+
+
+def new_gradients(count: int, device, viewspace: Optional["GaussianGradients"] = None):
+    """Uninitialised gradient planes for a backward that writes them
+    (LUMEN_EX_GRAD_ASSIGN), with the caller's view-space accumulators if any:
+    (planes, sh, LumenExGradients)."""
+    planes = torch.empty((11, count), dtype=torch.float32, device=device)
+    sh = torch.empty((count, 48), dtype=torch.float32, device=device)
+    vs = viewspace.viewspace_grad_2d if viewspace is not None else None
+    cnt = viewspace.observation_count if viewspace is not None else None
+    return planes, sh, gradients_struct(planes, sh, vs, cnt)
 
 
 class RasterizeSplats(torch.autograd.Function):
@@ -98,17 +116,15 @@ class RasterizeSplats(torch.autograd.Function):
         if R.frame_id != ctx.frame:
             # another render used this context since the forward: redo it
             R.render(ctx.scene, ctx.camera, ctx.config, save_pixels=True)
-        grads = GaussianGradients(ctx.scene.count, ctx.scene.device)
-        if ctx.viewspace is not None:
-            # densification statistics accumulate across views in the caller's object
-            grads.viewspace_grad_2d = ctx.viewspace.viewspace_grad_2d
-            grads.observation_count = ctx.viewspace.observation_count
+        # the backward writes every gradient entry (no zeroed planes needed);
+        # densification statistics accumulate across views in the caller's object
+        planes, sh, gs = new_gradients(ctx.scene.count, ctx.scene.device, ctx.viewspace)
         g_rgb = grad_rgb.contiguous() if grad_rgb is not None else \
             torch.zeros((ctx.camera.config.height, ctx.camera.config.width, 3),
                         device=ctx.scene.device)
         g_d = grad_depth.contiguous() if grad_depth is not None else None
-        R.render_backward(ctx.scene, ctx.camera, ctx.config, g_rgb, g_d, grads)
-        return grads.planes, grads.sh, None, None, None, None
+        R.render_backward(ctx.scene, ctx.camera, ctx.config, g_rgb, g_d, gs, assign=True)
+        return planes, sh, None, None, None, None
 
 
 def rasterize(planes: torch.Tensor, sh: torch.Tensor, camera: PinholeCamera,

@@ -231,3 +231,48 @@ def test_backward_is_deterministic(rasterizer):
     for r in results[1:]:
         for a, b in zip(results[0], r):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("views", [1, 3])
+def test_assign_equals_accumulate_into_zeros(rasterizer, views):
+    """LUMEN_EX_GRAD_ASSIGN writes exactly what accumulating into zeroed planes
+    gives, over planes full of garbage, including the zeros of Gaussians no
+    view sees (one view, and a batch of three); the view-space statistics
+    still accumulate."""
+    import torch
+    from lumen_amd.batch import MultiViewRenderer
+    from lumen_amd.camera import orbit
+    from lumen_amd.training import GaussianGradients, gradients_struct
+    scene = random_scene(40_000, 1.0, 97)
+    dev = scene.to_device()
+    cfg = RenderConfig()
+    rng = np.random.default_rng(8)
+    if views == 1:
+        cams = [look_at(CameraConfig(320, 240, 300.0, 300.0, 160.0, 120.0), (0.0, 0.5, 4.0))]
+        rasterizer.render(dev, cams[0], cfg, save_pixels=True)
+        run = lambda g, assign: rasterizer.render_backward(dev, cams[0], cfg, g_rgb, None, g,  # noqa
+                                                          assign=assign)
+    else:
+        cams = orbit(CameraConfig(320, 240, 300.0, 300.0, 160.0, 120.0), views, 4.5)
+        mv = MultiViewRenderer(0, rasterizer=rasterizer)
+        mv.render(dev, cams, cfg, save_pixels=True)
+        mv.synchronize()
+        run = lambda g, assign: (mv.backward(dev, cams, cfg, g_rgb, None, g, assign=assign),  # noqa
+                                 mv.synchronize())
+    P = sum(c.config.width * c.config.height for c in cams)
+    g_rgb = torch.from_numpy(rng.normal(0, 1e-3, 3 * P).astype(np.float32)).cuda()
+    if views == 1:
+        g_rgb = g_rgb.view(cams[0].config.height, cams[0].config.width, 3)
+    ref = GaussianGradients(scene.count)
+    run(ref, False)
+    planes = torch.full((11, scene.count), float("nan"), device="cuda")
+    sh = torch.full((scene.count, 48), 1234.5, device="cuda")
+    vs = GaussianGradients(scene.count)
+    vs.viewspace_grad_2d.fill_(1.0)
+    run(gradients_struct(planes, sh, vs.viewspace_grad_2d, vs.observation_count), True)
+    torch.cuda.synchronize()
+    assert torch.equal(planes, ref.planes) and torch.equal(sh, ref.sh)
+    assert torch.equal(vs.viewspace_grad_2d, ref.viewspace_grad_2d + 1.0)
+    assert torch.equal(vs.observation_count, ref.observation_count)
+    assert int((ref.observation_count == 0).sum()) > 0   # some Gaussians unseen
+    say(f"[bwd] assign == accumulate into zeros ({views} view(s))")
