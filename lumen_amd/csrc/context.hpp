@@ -102,7 +102,7 @@ struct GraphCache {
 
 struct LumenExContext {
   int device = 0;
-  lumen::DevBuf packed, part, totals, plan, views, spans, geo, block_bounds;
+  lumen::DevBuf packed, part, totals, plan, views, spans, segs, geo, block_bounds;
   lumen::DevBuf rec, vis2orig, radius;
   lumen::DevBuf vmeta;    // per visible splat: first key offset, tile rectangle (uint4)
   lumen::DevBuf keys[2], vals[2], keys_keep, vals_keep;

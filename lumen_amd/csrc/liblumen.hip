@@ -82,8 +82,8 @@ struct GeoCache {
 };
 
 // Pass 1: visibility and tile count per (view, Gaussian), the geometry of
-// the visible ones, and per-block bounds of depth keys and frame tile ids
-// (which decide the sort digits; reduced by k_reduce_bounds).
+// the visible ones, and per-block bounds of depth keys and view-local tile
+// ids (which decide the sort digits; reduced by k_reduce_bounds).
 __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
                                                int nviews, uint64_t* __restrict__ packed,
                                                GeoCache* __restrict__ geo,
@@ -108,8 +108,8 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
         const uint32_t dk = depth_key(g.depth);
         dmin = min(dmin, dk);
         dmax = max(dmax, dk);
-        tmin = min(tmin, v.tile_base + (uint32_t)(g.min_ty * v.tiles_x + g.min_tx));
-        tmax = max(tmax, v.tile_base + (uint32_t)(g.max_ty * v.tiles_x + g.max_tx));
+        tmin = min(tmin, (uint32_t)(g.min_ty * v.tiles_x + g.min_tx));  // view-local tiles
+        tmax = max(tmax, (uint32_t)(g.max_ty * v.tiles_x + g.max_tx));
         GeoCache gc;
         gc.a = make_float4(g.sx, g.sy, g.depth, g.c00);
         gc.b = make_float4(g.c01, g.c11,
@@ -186,8 +186,8 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict_
 __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
                        const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
                        uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
-                       ViewSpan* __restrict__ spans, int2* __restrict__ ranges,
-                       uint32_t tiles) {
+                       ViewSpan* __restrict__ spans, SortSeg* __restrict__ segs,
+                       int2* __restrict__ ranges, uint32_t tiles) {
   // every block: empty tile ranges (-1, -1) before k_ranges fills the used ones
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
     ranges[t] = make_int2(-1, -1);
@@ -210,6 +210,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
     plan->overflow = ((uint64_t)M > m_cap || (uint64_t)V > v_cap) ? 1u : 0u;
   }
+  const uint32_t mc = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
   for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
     const uint64_t a = offsets[(size_t)vw * n];
     const uint64_t b = vw + 1 < nviews ? offsets[(size_t)(vw + 1) * n] : tot;
@@ -219,6 +220,22 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     sp.key_base = (uint32_t)(a >> 32);
     sp.instances = (uint32_t)(b >> 32) - (uint32_t)(a >> 32);
     spans[vw] = sp;
+    // the view's keys as a sort segment, clipped to the key capacity
+    SortSeg sg;
+    sg.key_base = sp.key_base < mc ? sp.key_base : mc;
+    const uint32_t kend = sp.key_base + sp.instances < mc ? sp.key_base + sp.instances : mc;
+    sg.count = kend - sg.key_base;
+    sg.nblocks = (sg.count + kSortTile - 1) / kSortTile;
+    sg.block_base = 0;
+    segs[vw] = sg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // number the sort blocks through the segments
+    uint32_t b = 0;
+    for (int vw = 0; vw < nviews; ++vw) {
+      segs[vw].block_base = b;
+      b += segs[vw].nblocks;
+    }
   }
 }
 
@@ -284,7 +301,8 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
     pz = s.pz[i];
     op = s.op[i];
   }
-  // mode 1 plans sort 32-bit frame keys (tile - tmin) << dbits | (depth key - dmin)
+  // keys hold the view-local tile: mode 1 plans sort 32-bit frame keys
+  // (tile - tmin) << dbits | (depth key - dmin), mode 0 tile << 32 | depth key
   const bool frame_keys = plan->mode == 1;
   const uint32_t kt0 = plan->tmin, kd0 = plan->dmin;
   const int kdb = plan->dbits;
@@ -382,7 +400,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       const uint32_t row = local / wtiles;
       const uint32_t ty = min_ty + row;
       const uint32_t tx = min_tx + (local - row * wtiles);
-      const uint32_t tile = v.tile_base + ty * (uint32_t)v.tiles_x + tx;
+      const uint32_t tile = ty * (uint32_t)v.tiles_x + tx;  // within the view
       if (frame_keys)
         keys32[base + p] = ((tile - kt0) << kdb) | (s_dk[w][lo] - kd0);
       else
@@ -395,36 +413,77 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
   }
 }
 
-// Tile of sorted key i: the high half of a 64-bit key (mode 0) or the high
-// bits of a 32-bit frame key plus tmin (mode 1).
-__device__ __forceinline__ uint32_t sorted_tile(const void* keys, uint32_t i, bool frame_keys,
-                                                uint32_t tmin, int dbits) {
-  if (frame_keys) return (static_cast<const uint32_t*>(keys)[i] >> dbits) + tmin;
-  return (uint32_t)(static_cast<const uint64_t*>(keys)[i] >> 32);
+// The segment (view) holding key position i.
+__device__ __forceinline__ int seg_of(const SortSeg* __restrict__ segs, int nseg, uint32_t i) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].key_base <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
+__device__ __forceinline__ uint64_t sorted_key(const void* keys, uint32_t i, bool frame_keys) {
+  if (frame_keys) return static_cast<const uint32_t*>(keys)[i];
+  return static_cast<const uint64_t*>(keys)[i];
+}
+
+// Tile ranges from the sorted keys.  A run of one tile starts where the
+// view-local tile changes or a view's segment begins; the thread at a run's
+// start also closes the previous run, and the last key closes the final
+// one.  The previous key comes from the lane below (lane 0 loads it).
 __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
                                                 const uint64_t* __restrict__ k1,
                                                 const SortPlan* __restrict__ plan,
+                                                const SortSeg* __restrict__ segs, int nseg,
+                                                const ViewParams* __restrict__ views,
                                                 int2* __restrict__ ranges, uint32_t tiles) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const uint32_t m = plan->m;
-  if (i >= m || plan->overflow) return;
+  if (blockIdx.x * 256u >= m || plan->overflow) return;  // uniform across the block
+  const bool valid = i < m;
   const void* keys = (plan->passes & 1) ? (const void*)k1 : (const void*)k0;
-  const bool fk = plan->mode == 1;
-  const uint32_t tmin = plan->tmin;
-  const int db = plan->dbits;
-  const uint32_t tile = sorted_tile(keys, i, fk, tmin, db);
-  if (tile >= tiles) return;
-  if (i == 0 || sorted_tile(keys, i - 1, fk, tmin, db) != tile) ranges[tile].x = (int32_t)i;
-  if (i == m - 1 || sorted_tile(keys, i + 1, fk, tmin, db) != tile) ranges[tile].y = (int32_t)(i + 1);
+  const SortPlan p = *plan;
+  const bool fk = p.mode == 1;
+  const uint64_t k = valid ? sorted_key(keys, i, fk) : 0ull;
+  uint64_t kp = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(k >> 32), 1, 64) << 32) |
+                (uint32_t)__shfl_up((int)(uint32_t)k, 1, 64);
+  if ((threadIdx.x & 63) == 0 && valid && i > 0) kp = sorted_key(keys, i - 1, fk);
+  if (!valid) return;
+  // the segment of the block's first key (a search on uniform values), then
+  // forward to this key's
+  int sv = seg_of(segs, nseg, blockIdx.x * 256u);
+  while (sv + 1 < nseg && segs[sv + 1].key_base <= i) ++sv;
+  const bool seg_start = i == segs[sv].key_base;
+  const uint32_t tile = frame_key_tile(k, p);
+  const uint32_t gt = views[sv].tile_base + tile;
+  if (i == 0 || seg_start || frame_key_tile(kp, p) != tile) {
+    if (gt < tiles) ranges[gt].x = (int32_t)i;
+    if (i > 0) {  // close the previous run, in its own view
+      int sp = sv;
+      if (seg_start) {
+        sp = sv - 1;
+        while (sp > 0 && segs[sp].count == 0) --sp;
+      }
+      const uint32_t gp = views[sp].tile_base + frame_key_tile(kp, p);
+      if (gp < tiles) ranges[gp].y = (int32_t)i;
+    }
+  }
+  if (i + 1 == m && gt < tiles) ranges[gt].y = (int32_t)m;
 }
 
-// 32-bit frame keys back to the 64-bit keys (for lumen_ex_copy_stage).
-__global__ void k_decode_keys(const uint32_t* __restrict__ k32, uint32_t n,
-                              const SortPlan* __restrict__ plan, uint64_t* __restrict__ out) {
+// Frame keys (either width) back to the reference's 64-bit keys, for
+// lumen_ex_copy_stage.
+__global__ void k_decode_keys(const void* __restrict__ keys, uint32_t n,
+                              const SortPlan* __restrict__ plan, const SortSeg* __restrict__ segs,
+                              int nseg, const ViewParams* __restrict__ views,
+                              uint64_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) out[i] = frame_key_to_key(k32[i], *plan);
+  if (i >= n) return;
+  const SortPlan p = *plan;
+  const uint64_t k = sorted_key(keys, i, p.mode == 1);
+  out[i] = frame_key_to_key(k, p, views[seg_of(segs, nseg, i)].tile_base);
 }
 
 // Ranges of an arbitrary sorted key array (lumen_ex_build_ranges).
@@ -616,7 +675,12 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
   }
 }
 
-__global__ void k_set_plan(SortPlan* plan, int passes, const int32_t* digits, uint32_t m) {
+__global__ void k_set_plan(SortPlan* plan, SortSeg* seg, int passes, const int32_t* digits,
+                           uint32_t m) {
+  seg->key_base = 0;  // one segment: all the keys
+  seg->count = m;
+  seg->block_base = 0;
+  seg->nblocks = (m + kSortTile - 1) / kSortTile;
   plan->passes = passes;
   for (int k = 0; k < 8; ++k) {
     plan->shift[k] = k < passes ? 8 * digits[k] : 0;
@@ -714,6 +778,7 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   memcpy(c->pinned_views, c->host_views.data(), bytes);
   LUMEN_TRY(ensure(c->views, bytes));
   LUMEN_TRY(ensure(c->spans, sizeof(ViewSpan) * n));
+  LUMEN_TRY(ensure(c->segs, sizeof(SortSeg) * n));
   LUMEN_HIP(hipMemcpyAsync(c->views.ptr, c->pinned_views, bytes, hipMemcpyHostToDevice, st));
   return LUMEN_SUCCESS;
 }
@@ -840,8 +905,8 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
     const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 256 * 16);
     hipLaunchKernelGGL(k_plan, dim3(pb), dim3(256), 0, st, c->packed.as<uint64_t>(),
                        c->totals.as<uint64_t>(), frame_bounds(c), sd.n, (int)fs.nviews, m_cap,
-                       v_cap, plan, c->spans.as<ViewSpan>(), c->ranges.as<int2>(),
-                       (uint32_t)fs.tiles);
+                       v_cap, plan, c->spans.as<ViewSpan>(), c->segs.as<SortSeg>(),
+                       c->ranges.as<int2>(), (uint32_t)fs.tiles);
     LUMEN_LAUNCHED();
   } else {
     LUMEN_TRY(fill<uint32_t>(c->spans.as<uint32_t>(), 0u, 4ull * fs.nviews, st));
@@ -868,11 +933,13 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
     uint64_t* kb[2] = {c->keys[0].as<uint64_t>(), c->keys[1].as<uint64_t>()};
     uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
-    LUMEN_TRY(radix_sort_planned(kb, vb, plan, m_cap, sort_slots, sort_modes,
+    LUMEN_TRY(radix_sort_planned(kb, vb, plan, c->segs.as<SortSeg>(), (int)fs.nviews, m_cap,
+                                 sort_slots, sort_modes,
                                  SortScratch{&c->hist, &c->hist_part}, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
     hipLaunchKernelGGL(k_ranges, dim3(div_up(m_cap, 256)), dim3(256), 0, st, kb[0], kb[1], plan,
+                       c->segs.as<SortSeg>(), (int)fs.nviews, c->views.as<ViewParams>(),
                        c->ranges.as<int2>(), (uint32_t)fs.tiles);
     LUMEN_LAUNCHED();
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, true, st));
@@ -900,7 +967,13 @@ static int max_sort_slots(const LumenExContext* c, const FrameShape& fs) {
   const uint32_t dmax = depth_key(far_max);
   int32_t dg[8];
   int n = varying_digits(0, dmax, 0, 4, dg);
-  n += varying_digits(0, fs.tiles ? fs.tiles - 1 : 0, 4, 4, dg + n);
+  uint32_t tiles = 0;  // keys hold view-local tiles
+  for (const ViewParams& v : c->host_views) {
+    const uint32_t t = (uint32_t)v.tiles_x * (uint32_t)v.tiles_y;
+    tiles = t > tiles ? t : tiles;
+  }
+  (void)fs;
+  n += varying_digits(0, tiles ? tiles - 1 : 0, 4, 4, dg + n);
   return n;
 }
 
@@ -1013,8 +1086,8 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
   // sort scratch for the capacity, allocated before capture
-  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap) * sizeof(uint32_t)));
-  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_cap)));
+  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap, (int)nviews) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_cap, (int)nviews)));
   LUMEN_HIP(hipStreamSynchronize(st));
   const int slots = max_sort_slots(c, fs);
   const SceneDev sd = scene_dev(sc);
@@ -1060,7 +1133,7 @@ static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
           &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
           &c->hist,   &c->hist_part, &c->ranges,   &c->t_final,   &c->n_eval,   &c->minmax,
           &c->partial, &c->grad_vis, &c->sum_seg, &c->sum_aux, &c->loss_part, &c->dens_cls,
-          &c->dens_cnt, &c->dens_off, &c->dens_misc};
+          &c->dens_cnt, &c->dens_off, &c->dens_misc, &c->segs};
 }
 
 struct WorkspaceShape {
@@ -1102,11 +1175,12 @@ static LumenStatus workspace(LumenExContext* c, Need&& need, uint64_t N, const W
   const bool save = backward || (flags & LUMEN_EX_SAVE_PIXELS);
   LUMEN_TRY(need(c->views, sizeof(ViewParams) * ws.fs.nviews));
   LUMEN_TRY(need(c->spans, sizeof(ViewSpan) * ws.fs.nviews));
+  LUMEN_TRY(need(c->segs, sizeof(SortSeg) * ws.fs.nviews));
   LUMEN_TRY(frame_buffers(c, need, N, ws.fs, save));
   LUMEN_TRY(capacity_buffers(c, need, v_cap, m_cap));
   if (m_cap > 0) {
-    LUMEN_TRY(need(c->hist, sort_hist_len(m_cap) * sizeof(uint32_t)));
-    LUMEN_TRY(need(c->hist_part, sort_part_bytes(m_cap)));
+    LUMEN_TRY(need(c->hist, sort_hist_len(m_cap, (int)ws.fs.nviews) * sizeof(uint32_t)));
+    LUMEN_TRY(need(c->hist_part, sort_part_bytes(m_cap, (int)ws.fs.nviews)));
   }
   if ((flags & LUMEN_EX_KEEP_STAGES) && m_cap > 0) {
     LUMEN_TRY(need(c->keys_keep, m_cap * 8));
@@ -1338,17 +1412,18 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   LUMEN_TRY(ensure(c->keys[1], n * 8));
   LUMEN_TRY(ensure(c->vals[1], n * 4));
   LUMEN_TRY(ensure(c->minmax, 16 + 32));
+  LUMEN_TRY(ensure(c->segs, sizeof(SortSeg)));
   int32_t* ddig = reinterpret_cast<int32_t*>(static_cast<uint8_t*>(c->minmax.ptr) + 16);
   memcpy(static_cast<uint8_t*>(c->pinned) + 64, digits, sizeof(digits));
   LUMEN_HIP(hipMemcpyAsync(ddig, static_cast<uint8_t*>(c->pinned) + 64, sizeof(digits),
                            hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_set_plan, dim3(1), dim3(1), 0, st, c->plan.as<SortPlan>(), nd, ddig,
-                     (uint32_t)n);
+  hipLaunchKernelGGL(k_set_plan, dim3(1), dim3(1), 0, st, c->plan.as<SortPlan>(),
+                     c->segs.as<SortSeg>(), nd, ddig, (uint32_t)n);
   LUMEN_LAUNCHED();
   uint64_t* kb[2] = {keys, c->keys[1].as<uint64_t>()};
   uint32_t* vb[2] = {values, c->vals[1].as<uint32_t>()};
-  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), n, nd, 1,
-                               SortScratch{&c->hist, &c->hist_part}, st));
+  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), c->segs.as<SortSeg>(), 1, n, nd,
+                               1, SortScratch{&c->hist, &c->hist_part}, st));
   if (nd & 1) {
     LUMEN_HIP(hipMemcpyAsync(keys, kb[1], n * 8, hipMemcpyDeviceToDevice, st));
     LUMEN_HIP(hipMemcpyAsync(values, vb[1], n * 4, hipMemcpyDeviceToDevice, st));
@@ -1387,18 +1462,21 @@ LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint3
 // the sorted keys; for the kept unsorted keys, into = -1 takes whichever
 // buffer does not hold the sorted result) and points *src there.  Mode 0
 // leaves *src as it is.
-static LumenStatus decode_frame_keys(LumenExContext* c, const void* k32, int into,
+// The frame's keys in `keys` (either width, view-local tiles) as the
+// reference's 64-bit keys, written to ping-pong buffer `into` (-1: the one
+// not holding the sorted result).
+static LumenStatus decode_frame_keys(LumenExContext* c, const void* keys, int into,
                                      const void** src) {
   SortPlan p;
   LUMEN_HIP(hipSetDevice(c->device));
   LUMEN_HIP(hipDeviceSynchronize());
   LUMEN_HIP(hipMemcpy(&p, c->plan.ptr, sizeof(p), hipMemcpyDeviceToHost));
-  if (p.mode != 1) return LUMEN_SUCCESS;
   if (into < 0) into = 1 - (p.passes & 1);
   uint64_t* out = c->keys[into].as<uint64_t>();
-  hipLaunchKernelGGL(k_decode_keys, dim3(div_up(c->M, 256)), dim3(256), 0, nullptr,
-                     static_cast<const uint32_t*>(k32), (uint32_t)c->M, c->plan.as<SortPlan>(),
-                     out);
+  if (keys == out) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "decode_frame_keys: in place");
+  hipLaunchKernelGGL(k_decode_keys, dim3(div_up(c->M, 256)), dim3(256), 0, nullptr, keys,
+                     (uint32_t)c->M, c->plan.as<SortPlan>(), c->segs.as<SortSeg>(),
+                     (int)c->nviews, c->views.as<ViewParams>(), out);
   LUMEN_LAUNCHED();
   LUMEN_HIP(hipDeviceSynchronize());
   *src = out;

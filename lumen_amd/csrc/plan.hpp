@@ -14,8 +14,10 @@ constexpr int kDigitBins = 1 << kMaxDigitBits;   // histogram rows per block
 // of lumen_ex_radix_sort), read by every later kernel of the frame, so that a
 // frame needs no host round trip once its buffers are sized.
 //
-// mode 0 sorts the 64-bit keys tile << 32 | depth_key in 8-bit digits.
-// mode 1 sorts 32-bit frame keys (tile - tmin) << dbits | (depth_key - dmin),
+// Frame keys hold the tile id within the key's view (the sort keeps views
+// apart, see SortSeg).  mode 0 sorts the 64-bit keys tile << 32 | depth_key
+// in 8-bit digits.  mode 1 sorts 32-bit keys (tile - tmin) << dbits |
+// (depth_key - dmin),
 // used when the tile ids and depth keys of the frame vary in at most 32 bits
 // together: both transforms are monotone and the fields do not overlap, so
 // equal 64-bit keys map to equal frame keys and the stable order is the same.
@@ -30,6 +32,20 @@ struct SortPlan {
   uint32_t tmin, dmin; // frame-key offsets (mode 1)
   int32_t dbits;       // depth bits of a frame key (mode 1)
   uint32_t pad;
+};
+
+// One segment of a frame's keys: a view's contiguous run (emission is
+// view-major) sorted on its own.  The sort's blocks are numbered through the
+// segments (block_base), each block ranking keys of one segment only, so
+// the view grouping survives every pass and the keys need only the view's
+// own tile ids.  The histogram table is segment-major, then digit, then the
+// segment's blocks, so one flat exclusive scan gives every (segment, digit,
+// block) its output offset.
+struct SortSeg {
+  uint32_t key_base;    // first key of the segment
+  uint32_t count;       // keys in the segment
+  uint32_t block_base;  // first sort block of the segment
+  uint32_t nblocks;     // sort blocks of the segment
 };
 
 // Digits that can differ between keys lying in [lo, hi]: those at or below the
@@ -93,11 +109,21 @@ __host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dm
   }
 }
 
-// A frame key (mode 1) back to the 64-bit key.
-__host__ __device__ inline uint64_t frame_key_to_key(uint32_t k, const SortPlan& p) {
+// The view-local tile of a frame key: mode 1 (k >> dbits) + tmin, mode 0
+// the high half.
+__host__ __device__ inline uint32_t frame_key_tile(uint64_t k, const SortPlan& p) {
+  if (p.mode != 1) return (uint32_t)(k >> 32);
+  return (p.dbits < 32 ? ((uint32_t)k >> p.dbits) : 0u) + p.tmin;
+}
+
+// A frame key of a view whose tiles start at tile_base back to the 64-bit
+// key tile << 32 | depth key of the reference.
+__host__ __device__ inline uint64_t frame_key_to_key(uint64_t k, const SortPlan& p,
+                                                     uint32_t tile_base) {
+  const uint32_t tile = frame_key_tile(k, p) + tile_base;
+  if (p.mode != 1) return ((uint64_t)tile << 32) | (k & 0xFFFFFFFFull);
   const uint32_t dmask = p.dbits ? (0xFFFFFFFFu >> (32 - p.dbits)) : 0u;
-  const uint32_t tile = (p.dbits < 32 ? (k >> p.dbits) : 0u) + p.tmin;
-  return ((uint64_t)tile << 32) | (uint64_t)((k & dmask) + p.dmin);
+  return ((uint64_t)tile << 32) | (uint64_t)(((uint32_t)k & dmask) + p.dmin);
 }
 
 }  // namespace lumen

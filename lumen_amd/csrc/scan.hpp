@@ -17,9 +17,19 @@ constexpr int kScanChunk = kScanThreads * kScanItems;
 
 __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
 
+// A scan that runs only when *value > above (value null: always), so that a
+// graph can hold scans of sort passes the frame's device plan may not use.
+struct ScanGate {
+  const int32_t* value = nullptr;
+  int32_t above = 0;
+};
+
+__device__ __forceinline__ bool gate_closed(ScanGate g) { return g.value && !(*g.value > g.above); }
+
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restrict__ in, uint64_t n,
-                                                              T* __restrict__ partials) {
+                                                              T* __restrict__ partials, ScanGate gate) {
+  if (gate_closed(gate)) return;
   __shared__ T s_w[kScanThreads / 64];
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
   T s = 0;
@@ -41,7 +51,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restric
 
 template <typename T>
 __global__ __launch_bounds__(1024) void k_scan_spine(T* __restrict__ partials, uint32_t nb,
-                                                     T* __restrict__ total_out) {
+                                                     T* __restrict__ total_out, ScanGate gate) {
+  if (gate_closed(gate)) return;
   __shared__ T s_w[1024 / 64];
   T carry = 0;
   for (uint32_t base = 0; base < nb; base += 1024) {
@@ -58,7 +69,8 @@ __global__ __launch_bounds__(1024) void k_scan_spine(T* __restrict__ partials, u
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict__ in, uint64_t n,
                                                             const T* __restrict__ partials,
-                                                            T* __restrict__ out) {
+                                                            T* __restrict__ out, ScanGate gate) {
+  if (gate_closed(gate)) return;
   __shared__ T s[kScanChunk + kScanChunk / 32];
   __shared__ T s_w[kScanThreads / 64];
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
@@ -103,7 +115,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
                                                                   uint64_t n,
                                                                   const T* __restrict__ partials,
                                                                   T* __restrict__ out,
-                                                                  T* __restrict__ total_out) {
+                                                                  T* __restrict__ total_out, ScanGate gate) {
+  if (gate_closed(gate)) return;
   __shared__ T s[kScanChunk + kScanChunk / 32];
   __shared__ T s_w[kScanThreads / 64];
   __shared__ T s_prefix;
@@ -160,19 +173,21 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
 // the sum of all elements.
 template <typename T>
 inline LumenStatus exclusive_scan(const T* in, T* out, uint64_t n, T* partials, T* total_out,
-                                  hipStream_t st) {
+                                  hipStream_t st, ScanGate gate = ScanGate{}) {
   const uint32_t nb = div_up(n ? n : 1, kScanChunk);
-  hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials);
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials,
+                     gate);
   LUMEN_LAUNCHED();
   if (nb <= kScanFuse) {
     hipLaunchKernelGGL(k_scan_down_fused<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials,
-                       out, total_out);
+                       out, total_out, gate);
     LUMEN_LAUNCHED();
     return LUMEN_SUCCESS;
   }
-  hipLaunchKernelGGL(k_scan_spine<T>, dim3(1), dim3(1024), 0, st, partials, nb, total_out);
+  hipLaunchKernelGGL(k_scan_spine<T>, dim3(1), dim3(1024), 0, st, partials, nb, total_out, gate);
   LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials, out);
+  hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials, out,
+                     gate);
   LUMEN_LAUNCHED();
   return LUMEN_SUCCESS;
 }

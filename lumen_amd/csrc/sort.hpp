@@ -10,8 +10,9 @@
 // host round trip; grids are sized for a capacity and surplus blocks exit.
 // Each pass:
 //   k_sort_hist     per-block histogram of the digit (LDS atomics on
-//                   per-wave sub-histograms), stored digit-major so that one
-//                   exclusive scan gives every block's start offset per digit
+//                   per-wave sub-histograms), stored segment-major then
+//                   digit-major (SortSeg) so that one exclusive scan gives
+//                   every block's start offset per digit
 //   exclusive_scan  over the kDigitBins x blocks table
 //   k_sort_scatter  stable rank inside the block: each wave owns 1024
 //                   consecutive keys and ranks them 64 at a time with one
@@ -33,8 +34,27 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
-__device__ __forceinline__ uint32_t sort_blocks(uint32_t m) {
-  return (m + kSortTile - 1) / kSortTile;
+// The segment and segment-local index of sort block b; false for a surplus
+// block of a capacity-sized grid.  Segments without keys have no blocks (the
+// search takes the last segment starting at or before b).
+__device__ __forceinline__ bool sort_block(const SortSeg* __restrict__ segs, int nseg, uint32_t b,
+                                           SortSeg& sg, uint32_t& lb) {
+  const SortSeg last = segs[nseg - 1];
+  if (b >= last.block_base + last.nblocks) return false;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].block_base <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  sg = segs[lo];
+  lb = b - sg.block_base;
+  return true;
+}
+
+// Position of (digit d, block lb) of segment sg in the histogram table.
+__device__ __forceinline__ size_t hist_index(const SortSeg& sg, uint32_t d, uint32_t lb) {
+  return (size_t)sg.block_base * kDigitBins + (size_t)d * sg.nblocks + lb;
 }
 
 // K = uint64_t sorts mode-0 plans (64-bit keys), K = uint32_t mode-1 plans
@@ -48,37 +68,37 @@ __device__ __forceinline__ bool sort_mode_is(const SortPlan* plan) {
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K* k1,
                                                            const SortPlan* __restrict__ plan,
-                                                           int pass, uint32_t* __restrict__ hist,
-                                                           uint32_t nb_cap) {
+                                                           const SortSeg* __restrict__ segs,
+                                                           int nseg, int pass,
+                                                           uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[4][kDigitBins];
   if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
-  const uint32_t m = plan->m;
+  SortSeg sg;
+  uint32_t lb;
+  if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
   const int t = threadIdx.x;
-  if (blockIdx.x >= sort_blocks(m)) {  // surplus block of a capacity-sized grid
-    for (int d = t; d < kDigitBins; d += kSortThreads) hist[(uint32_t)d * nb_cap + blockIdx.x] = 0u;
-    return;
-  }
   const K* __restrict__ keys = (pass & 1) ? k1 : k0;
   const int shift = plan->shift[pass];
   const uint32_t mask = (1u << plan->width[pass]) - 1u;
   const int w = t >> 6;
   for (int d = t; d < 4 * kDigitBins; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
-  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t base = sg.key_base + lb * kSortTile;
+  const uint32_t end = sg.key_base + sg.count;
 #pragma unroll 4
   for (int j = 0; j < kSortItems; ++j) {
     const uint32_t idx = base + j * kSortThreads + t;
-    if (idx < m) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & mask], 1u);
+    if (idx < end) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & mask], 1u);
   }
   __syncthreads();
   for (int d = t; d < kDigitBins; d += kSortThreads)
-    hist[(uint32_t)d * nb_cap + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    hist[hist_index(sg, (uint32_t)d, lb)] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
 template <typename K>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
-    K* k0, K* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan, int pass,
-    const uint32_t* __restrict__ offsets, uint32_t nb_cap) {
+    K* k0, K* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
+    const SortSeg* __restrict__ segs, int nseg, int pass, const uint32_t* __restrict__ offsets) {
   __shared__ K s_key[kSortTile];
   __shared__ uint32_t s_val[kSortTile];
   __shared__ uint32_t s_cnt[4][kDigitBins];  // per-wave digit counts, then per-wave starts
@@ -86,8 +106,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   __shared__ uint32_t s_glob[kDigitBins];    // global start of each digit for this block
   __shared__ uint32_t s_w[4];
   if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
-  const uint32_t m = plan->m;
-  if (blockIdx.x >= sort_blocks(m)) return;
+  SortSeg sg;
+  uint32_t lb;
+  if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
+  const uint32_t m = sg.key_base + sg.count;  // end of this block's segment
   const K* __restrict__ kin = (pass & 1) ? k1 : k0;
   const uint32_t* __restrict__ vin = (pass & 1) ? v1 : v0;
   K* __restrict__ kout = (pass & 1) ? k0 : k1;
@@ -100,13 +122,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const int lane = t & 63;
   const uint64_t lt = lanemask_lt();
   for (int d = t; d < kDigitBins; d += kSortThreads) {
-    s_glob[d] = offsets[(uint32_t)d * nb_cap + blockIdx.x];
+    s_glob[d] = offsets[hist_index(sg, (uint32_t)d, lb)];
 #pragma unroll
     for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0u;
   }
   __syncthreads();
   // wave w owns keys [base + 1024 w, base + 1024 (w + 1)), 64 per round
-  const uint32_t base = blockIdx.x * kSortTile;
+  const uint32_t base = sg.key_base + lb * kSortTile;
   const uint32_t wbase = base + (uint32_t)w * (kSortTile / 4);
   K kk[kSortItems];
   uint32_t vv[kSortItems];
@@ -190,13 +212,16 @@ struct SortScratch {
   DevBuf* hist_part;
 };
 
-// Histogram rows (one per digit per block) and the scan's partial sums for
-// sorting up to m_cap keys.
-inline uint64_t sort_hist_len(uint64_t m_cap) {
-  return (uint64_t)div_up(m_cap, kSortTile) * kDigitBins;
+// Sort blocks (the grid) for up to m_cap keys in nseg segments, the
+// histogram table (one row per digit per block) and the scan's partial sums.
+inline uint32_t sort_grid(uint64_t m_cap, int nseg) {
+  return div_up(m_cap, kSortTile) + (uint32_t)nseg;
 }
-inline size_t sort_part_bytes(uint64_t m_cap) {
-  return (size_t)div_up(sort_hist_len(m_cap), kScanChunk) * sizeof(uint32_t);
+inline uint64_t sort_hist_len(uint64_t m_cap, int nseg) {
+  return (uint64_t)sort_grid(m_cap, nseg) * kDigitBins;
+}
+inline size_t sort_part_bytes(uint64_t m_cap, int nseg) {
+  return (size_t)div_up(sort_hist_len(m_cap, nseg), kScanChunk) * sizeof(uint32_t);
 }
 
 // Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
@@ -206,36 +231,38 @@ inline size_t sort_part_bytes(uint64_t m_cap) {
 // cannot know the frame's mode, takes both).  m_cap bounds the number of
 // keys the plan may name.
 inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
-                                      uint64_t m_cap, int slots, int modes, SortScratch sc,
-                                      hipStream_t st) {
+                                      const SortSeg* segs, int nseg, uint64_t m_cap, int slots,
+                                      int modes, SortScratch sc, hipStream_t st) {
   if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
-  const uint32_t nb = div_up(m_cap, kSortTile);
-  const uint64_t hlen = sort_hist_len(m_cap);
+  const uint32_t nb = sort_grid(m_cap, nseg);
+  const uint64_t hlen = sort_hist_len(m_cap, nseg);
   LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
-  LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap)));
+  LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap, nseg)));
   uint32_t* hist = sc.hist->as<uint32_t>();
   uint32_t* part = sc.hist_part->as<uint32_t>();
   uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};
   for (int p = 0; p < slots; ++p) {
     if (modes & 1) {
       hipLaunchKernelGGL(k_sort_hist<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1],
-                         plan, p, hist, nb);
+                         plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
     if (modes & 2) {
       hipLaunchKernelGGL(k_sort_hist<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
-                         k32[1], plan, p, hist, nb);
+                         k32[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
-    LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st));
+    // the pass's scan runs only when the device plan has the pass
+    LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st,
+                                       ScanGate{&plan->passes, p}));
     if (modes & 1) {
       hipLaunchKernelGGL(k_sort_scatter<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0],
-                         k[1], v[0], v[1], plan, p, hist, nb);
+                         k[1], v[0], v[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
     if (modes & 2) {
       hipLaunchKernelGGL(k_sort_scatter<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
-                         k32[1], v[0], v[1], plan, p, hist, nb);
+                         k32[1], v[0], v[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
   }
