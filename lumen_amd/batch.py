@@ -92,13 +92,16 @@ class MultiViewRenderer:
     def render(self, scene: DeviceScene, cameras: Sequence[PinholeCamera],
                config: Optional[RenderConfig] = None, graph: bool = False,
                save_pixels: bool = False, want_stats: bool = False,
-               out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None) -> BatchOutput:
+               out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
+               keep_stages: bool = False) -> BatchOutput:
         """Enqueue one frame on self.stream (the caller's current stream is
         made to wait for the scene first).  want_stats synchronizes."""
         config = config or RenderConfig()
         n = len(cameras)
         cams = (_lib.LumenExCamera * n)(*[camera_struct(c) for c in cameras])
-        flags = (_lib.LUMEN_EX_GRAPH if graph else 0) | (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0)
+        flags = (_lib.LUMEN_EX_GRAPH if graph else 0) | \
+            (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0) | \
+            (_lib.LUMEN_EX_KEEP_STAGES if keep_stages else 0)
         cfg = config.ex_struct(flags)
         if out is not None:
             rgb, depth = out

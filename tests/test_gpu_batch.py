@@ -188,3 +188,50 @@ def test_batch_autograd_releases_memory(renderer):
         grown = torch.cuda.memory_allocated() - base
         say(f"[batch] autograd memory growth over 6 steps: {grown} B")
         assert grown < 1 << 20
+
+
+def test_batch_keys_and_ranges_per_view(renderer, rasterizer):
+    """Each view of a batched frame is sorted as its own segment with
+    view-local tiles; decoded back, its keys (before and after the sort),
+    values and tile ranges equal a single-view render's with the view's key,
+    visible and tile offsets removed.  Mixed resolutions, and a view that sees
+    nothing (an empty segment) in the middle."""
+    from lumen_amd.raster import RenderConfig
+    scene = random_scene(60_000, 1.0, 99)
+    dev = scene.to_device()
+    cams = mixed_cameras()
+    away = look_at(CameraConfig(200, 150, 200.0, 200.0, 100.0, 75.0), (0.0, 0.0, 40.0),
+                   (0.0, 0.0, 80.0))  # looks away from the cloud
+    cams = cams[:2] + [away] + cams[2:]
+    cfg = RenderConfig()
+    out = renderer.render(dev, cams, cfg, want_stats=True, keep_stages=True)
+    renderer.synchronize()
+    st = out.stats
+    assert st[2].total_splat_instances == 0 and st[2].visible_splats == 0
+    R = renderer.rasterizer
+    M = sum(s.total_splat_instances for s in st)
+    T = sum(cfg.tiles(c.config.width, c.config.height)[0] * cfg.tiles(c.config.width, c.config.height)[1]
+            for c in cams)
+    keys_u = R.frame_stage("keys_unsorted", M, np.uint64)
+    vals_u = R.frame_stage("vals_unsorted", M, np.uint32)
+    keys = R.frame_stage("keys", M, np.uint64)
+    vals = R.frame_stage("vals", M, np.uint32)
+    ranges = R.frame_stage("ranges", 2 * T, np.int32).reshape(T, 2)
+    tile_base = 0
+    for k, cam in enumerate(cams):
+        tx, ty = cfg.tiles(cam.config.width, cam.config.height)
+        one = rasterizer.render(dev, cam, cfg, keep_stages=True)
+        kb, n, vb = st[k].key_base, st[k].total_splat_instances, st[k].vis_base
+        assert n == one.total_splat_instances
+        shift = np.uint64(tile_base) << np.uint64(32)
+        if n:
+            np.testing.assert_array_equal(keys_u[kb:kb + n] - shift, rasterizer.stage("keys_unsorted"))
+            np.testing.assert_array_equal(keys[kb:kb + n] - shift, rasterizer.stage("keys"))
+            np.testing.assert_array_equal(vals_u[kb:kb + n] - vb, rasterizer.stage("vals_unsorted"))
+            np.testing.assert_array_equal(vals[kb:kb + n] - vb, rasterizer.stage("vals"))
+        r1 = rasterizer.stage("ranges")
+        rb = ranges[tile_base:tile_base + tx * ty].copy()
+        rb[rb[:, 0] >= 0] -= kb
+        np.testing.assert_array_equal(rb, r1)
+        tile_base += tx * ty
+    say(f"[batch] per-view keys, values and ranges of {len(cams)} views (one empty) match")
