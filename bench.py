@@ -87,13 +87,16 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def stage_work(N, V, M, T, passes, E):
-    """Algorithmic bytes (or flops) per launch of each stage."""
+def stage_work(N, V, M, T, passes, E, kb):
+    """Algorithmic bytes (or flops) per launch of each stage; kb is the
+    width of the sorted keys (4 for 32-bit frame keys, 8 for 64-bit)."""
     return {
         "count": ("hbm", 68.0 * N + 32.0 * V),
-        "emit": ("hbm", 24.0 * N + 296.0 * V + 12.0 * M),
-        "sort": ("hbm", 8.0 * M + 24.0 * passes * M),
-        "ranges": ("hbm", 8.0 * M + 8.0 * T),
+        "emit": ("hbm", 24.0 * N + 296.0 * V + (kb + 4.0) * M),
+        # per pass: the histogram reads the keys, the scatter reads and
+        # writes keys and values
+        "sort": ("hbm", passes * (kb + 2.0 * (kb + 4.0)) * M),
+        "ranges": ("hbm", kb * M + 8.0 * T),
         "composite": ("valu", float(COMPOSITE_FLOPS_PER_EVAL) * E),
     }
 
@@ -135,6 +138,12 @@ def timed(fn, steps):
     return (time.perf_counter() - t0) / steps
 
 
+def timed_median(fn, steps, repeats=3):
+    """Median of `repeats` timed runs of `steps` calls (seconds per call)."""
+    ts = sorted(timed(fn, steps) for _ in range(repeats))
+    return ts[len(ts) // 2]
+
+
 # ---- headline: one 1M / 1080p frame ----------------------------------------
 
 def headline(args, result):
@@ -157,6 +166,7 @@ def headline(args, result):
     torch.cuda.synchronize()
     E = int(R.stage("n_eval").astype(np.int64).sum())
     V, M, passes = out.visible_splats, out.total_splat_instances, out.sort_passes
+    kb = out.key_bytes
     tx, ty = cfg.tiles(W, H)
     T = tx * ty
     N = scene.count
@@ -191,12 +201,16 @@ def headline(args, result):
     ms_events = (te1 - te0) * 1e3 / args.steps
 
     traffic = measured_traffic()
-    work = stage_work(N, V, M, T, passes, E)
+    work = stage_work(N, V, M, T, passes, E, kb)
     stages = {}
     for k, (b, w) in work.items():
         if stage_ms.get(k, 0) > 0:
             kern = KERNELS[k]
-            tr = traffic.get(kern) if "+" not in kern else None
+            if k == "sort":  # every pass runs a histogram and a scatter
+                th, ts = traffic.get("k_sort_hist"), traffic.get("k_sort_scatter")
+                tr = passes * (th + ts) if th and ts else None
+            else:
+                tr = traffic.get(kern)
             stages[k] = roofline_entry(b, w, stage_ms[k], int(tr) if tr else None)
     dominant = max(stages, key=lambda k: stages[k]["avg_ms"])
     roofline = dict(stages[dominant])
@@ -217,7 +231,7 @@ def headline(args, result):
         "config": {"workload": args.workload, "gaussians": N, "width": W, "height": H,
                    "tile": f"{cfg.tile_width}x{cfg.tile_height}", "sh_degree": cfg.max_sh_degree,
                    "mode": "forward", "visible": V, "instances": M, "evaluations": E,
-                   "sort_passes": passes},
+                   "sort_passes": passes, "sort_key_bytes": kb},
         "roofline": roofline,
         "stages": stages,
         "ms_per_step_with_stage_events": round(ms_events, 4),
@@ -282,7 +296,8 @@ def batch_leg(args):
     M = sum(s.total_splat_instances for s in stats)
     V = sum(s.visible_splats for s in stats)
     steps = max(3, args.steps // 6)
-    t_batch = timed(lambda: mv.render(dev, cams, cfg, graph=True), steps)
+    # the median of three timed runs: a frame of 32 1080p views takes ~35 ms
+    t_batch = timed_median(lambda: mv.render(dev, cams, cfg, graph=True), steps)
     batched_rgb = out.rgb.clone()
 
     R = mv.rasterizer
@@ -295,7 +310,7 @@ def batch_leg(args):
             R.render(dev, c, cfg, out_rgb=o_rgb, out_depth=o_d)
 
     plain()
-    t_plain = timed(plain, steps)
+    t_plain = timed_median(plain, steps)
     P = W * H
     same = all(torch.equal(outs[k][0].reshape(-1), batched_rgb[3 * k * P: 3 * (k + 1) * P])
                for k in range(nv))

@@ -90,8 +90,8 @@ typedef struct {
     uint64_t total_splat_instances; /* RenderOutput.total_splat_instances (pipeline.mojo:160) */
     uint64_t tiles_x;
     uint64_t tiles_y;
-    uint32_t sort_passes;           /* 8-bit digit passes actually run */
-    uint32_t reserved;
+    uint32_t sort_passes;           /* radix digit passes actually run */
+    uint32_t key_bytes;             /* 4: 32-bit frame keys were sorted, 8: 64-bit keys */
 } LumenExRenderStats;
 
 LumenExContext* lumen_ex_context_create(int device);

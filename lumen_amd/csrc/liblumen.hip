@@ -1023,6 +1023,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   c->tile_h = cfg->tile_height;
   c->v_cap = V;
   c->m_cap = M;
+  c->key_bytes = hp.mode == 1 ? 4u : 8u;
   c->sizes_known = true;
   c->kept = keep;
   c->pixels = save;
@@ -1328,7 +1329,7 @@ LumenStatus lumen_ex_render(LumenExContext* c, const LumenExScene* sc, const Lum
     stats->tiles_x = (uint64_t)c->host_views[0].tiles_x;
     stats->tiles_y = (uint64_t)c->host_views[0].tiles_y;
     stats->sort_passes = (uint32_t)passes;
-    stats->reserved = 0;
+    stats->key_bytes = c->key_bytes;
   }
   return LUMEN_SUCCESS;
 }

@@ -69,6 +69,7 @@ class RenderOutput:
     visible_splats: int
     total_splat_instances: int
     sort_passes: int = 0
+    key_bytes: int = 8
 
     def get_pixel_rgb(self, x: int, y: int):
         if 0 <= x < self.width and 0 <= y < self.height:
@@ -138,7 +139,8 @@ class Rasterizer:
         self._last = (int(stats.visible_splats), int(stats.total_splat_instances),
                       int(stats.tiles_x * stats.tiles_y), W * H)
         return RenderOutput(W, H, out_rgb, out_depth, int(stats.visible_splats),
-                            int(stats.total_splat_instances), int(stats.sort_passes))
+                            int(stats.total_splat_instances), int(stats.sort_passes),
+                            int(stats.key_bytes))
 
     def stage(self, name: str) -> np.ndarray:
         """Host copy of a stage buffer of the last render (see lumen_ex.h)."""
