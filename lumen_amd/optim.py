@@ -205,9 +205,9 @@ class TrainingCoordinator:
         self.config = train_cfg
         self.render_config = render_cfg or RenderConfig()
         self.rasterizer = rasterizer or default_rasterizer()
-        # gradients of the step; the view-space statistics in it accumulate
-        # over the steps until the next densification
-        self.grads: Optional[GaussianGradients] = None
+        # the view-space statistics, accumulated over the steps until the
+        # next densification
+        self.stats: Optional[GaussianGradients] = None
 
     def train_step(self, scene: DeviceScene, camera: PinholeCamera, ground_truth: torch.Tensor,
                    optimizer: AdamOptimizer, iteration: int, scene_radius: float
@@ -216,23 +216,23 @@ class TrainingCoordinator:
         reset when the schedule says so.  Returns the (possibly new) scene."""
         planes = scene.planes.detach().requires_grad_(True)
         sh = scene.sh.detach().requires_grad_(True)
-        if self.grads is None or self.grads.count != scene.count:
-            self.grads = GaussianGradients(scene.count, scene.device)
-        grads = self.grads
-        grads.zero()
+        if self.stats is None or self.stats.count != scene.count:
+            z = lambda dt: torch.zeros(scene.count, dtype=dt, device=scene.device)  # noqa: E731
+            self.stats = GaussianGradients.wrap(None, None, z(torch.float32), z(torch.int32))
         img, _ = rasterize(planes, sh, camera, self.render_config, rasterizer=self.rasterizer,
-                           viewspace=grads)
+                           viewspace=self.stats)
         loss = combined_loss(img, ground_truth, self.config.lambda_dssim, self.rasterizer)
         loss.backward()
-        grads.planes.copy_(planes.grad)
-        grads.sh.copy_(sh.grad)
+        # the backward's gradient planes as they are, with the statistics
+        grads = GaussianGradients.wrap(planes.grad, sh.grad, self.stats.viewspace_grad_2d,
+                                       self.stats.observation_count)
         loss_v = float(loss.detach())
         psnr_v = psnr(img.detach(), ground_truth)
         optimizer.step(scene, grads, self.config)
         if self.config.should_densify(iteration):
             scene, _ = densify_and_prune(scene, grads, self.config, scene_radius, optimizer,
                                          self.rasterizer)
-            self.grads = None
+            self.stats = None
         if self.config.should_reset_opacity(iteration):
             reset_opacities(scene, 0.01)
         return scene, TrainingMetrics(iteration, loss_v, psnr_v, scene.count)

@@ -30,9 +30,18 @@ class GaussianGradients:
         self.viewspace_grad_2d = torch.zeros(count, dtype=torch.float32, device=device)
         self.observation_count = torch.zeros(count, dtype=torch.int32, device=device)
 
+    @classmethod
+    def wrap(cls, planes, sh, viewspace_grad_2d, observation_count) -> "GaussianGradients":
+        """Gradients over existing tensors (no allocation); planes / sh may be
+        None for a holder of the view-space statistics only."""
+        g = cls.__new__(cls)
+        g.planes, g.sh = planes, sh
+        g.viewspace_grad_2d, g.observation_count = viewspace_grad_2d, observation_count
+        return g
+
     @property
     def count(self) -> int:
-        return int(self.planes.shape[1])
+        return int(self.viewspace_grad_2d.shape[0])
 
     def __getattr__(self, name):
         # grad_pos_x ... grad_opacity as views of the planes
