@@ -9,19 +9,22 @@
 //   scan         visible index and key offset per (view, Gaussian); stable,
 //                so each view's visible order is the reference's append
 //                order                               :141-142
-//   k_plan       sort digit plan, key count and per-view spans, on the device
+//   k_plan       sort digit plan, key count, per-view spans and sort
+//                segments, on the device
 //   k_emit       the same geometry (bit-identical), SH colour, inverse
 //                covariance, the 48-byte splat record at its visible index,
-//                and the 64-bit keys (frame tile id << 32 | depth)  :132-157
-//   sort         stable LSD radix sort over the digits that vary    :166
+//                and the keys: (tile within the view, depth key), 32-bit
+//                frame keys when the frame's ranges fit, else 64-bit  :132-157
+//   sort         stable LSD radix sort over the digits that vary, each
+//                view's keys a segment of its own                   :166
 //   k_ranges     per-tile [start, end)                               :169
 //   k_composite  one workgroup per tile, one lane per pixel, records staged
 //                through LDS in batches, per-pixel front-to-back blend with
 //                T <= 1e-4 termination and a block-wide exit         :172-229
 //
 // A view's tiles are frame tiles [tile_base, tile_base + tiles); with one
-// view the frame tile id is the reference's tile id, so keys, values and
-// ranges are the reference's.  The plain path blocks the host once per frame
+// view the frame tile id is the reference's tile id, so keys (decoded),
+// values and ranges are the reference's.  The plain path blocks the host once per frame
 // (to size the key buffers); the graph path (LUMEN_EX_GRAPH) captures the
 // whole frame once with reserved capacities and replays it.
 #include <hip/hip_runtime.h>
@@ -1458,14 +1461,11 @@ LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint3
   return LUMEN_SUCCESS;
 }
 
-// For a frame sorted as 32-bit frame keys (mode 1), decodes the c->M keys at
-// k32 into 64-bit keys in buffer keys[into] (the other ping-pong buffer for
-// the sorted keys; for the kept unsorted keys, into = -1 takes whichever
-// buffer does not hold the sorted result) and points *src there.  Mode 0
-// leaves *src as it is.
-// The frame's keys in `keys` (either width, view-local tiles) as the
-// reference's 64-bit keys, written to ping-pong buffer `into` (-1: the one
-// not holding the sorted result).
+// The frame's c->M keys at `keys` (either width, view-local tiles) as the
+// reference's 64-bit keys tile << 32 | depth key with frame tile ids, written
+// to ping-pong buffer keys[into] (for the sorted keys the other buffer; for
+// the kept unsorted keys, into = -1 takes whichever buffer does not hold the
+// sorted result); *src points there.
 static LumenStatus decode_frame_keys(LumenExContext* c, const void* keys, int into,
                                      const void** src) {
   SortPlan p;
