@@ -235,3 +235,40 @@ def test_batch_keys_and_ranges_per_view(renderer, rasterizer):
         np.testing.assert_array_equal(rb, r1)
         tile_base += tx * ty
     say(f"[batch] per-view keys, values and ranges of {len(cams)} views (one empty) match")
+
+
+def test_batch_wide_keys_and_many_views(renderer, rasterizer):
+    """Two regimes of the per-view sort segments: a frame whose keys need 64
+    bits (far depth range) and a frame of 40 small views; every view's image
+    and per-view keys equal its single-view render."""
+    import torch
+    from lumen_amd.raster import RenderConfig
+    cfg = RenderConfig()
+    wide = random_scene(30_000, 1000.0, 81)
+    wide.planes[3:6] *= 1000.0
+    small = random_scene(20_000, 1.0, 82)
+    cases = [
+        (wide, [look_at(CameraConfig(640, 480, 600.0, 600.0, 320.0, 240.0, 0.01, 1.0e6), eye)
+                for eye in ((0.0, 0.0, 5000.0), (3000.0, 500.0, 4000.0))], 8),
+        (small, orbit(CameraConfig(64, 48, 60.0, 60.0, 32.0, 24.0), 40, 4.0), None),
+    ]
+    for scene, cams, key_bytes in cases:
+        dev = scene.to_device()
+        out = renderer.render(dev, cams, cfg, want_stats=True, keep_stages=True)
+        renderer.synchronize()
+        M = sum(s.total_splat_instances for s in out.stats)
+        keys = renderer.rasterizer.frame_stage("keys", M, np.uint64)
+        tile_base = 0
+        for k, cam in enumerate(cams):
+            one = rasterizer.render(dev, cam, cfg, keep_stages=True)
+            if key_bytes:
+                assert one.key_bytes == key_bytes
+            torch.cuda.synchronize()
+            assert torch.equal(out.views_rgb[k], one.rgb), k
+            kb, n = out.stats[k].key_base, out.stats[k].total_splat_instances
+            if n:
+                np.testing.assert_array_equal(keys[kb:kb + n] - (np.uint64(tile_base) << np.uint64(32)),
+                                              rasterizer.stage("keys"))
+            tx, ty = cfg.tiles(cam.config.width, cam.config.height)
+            tile_base += tx * ty
+        say(f"[batch] {len(cams)} views, {M} keys: per-view images and keys match")
