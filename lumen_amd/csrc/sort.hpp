@@ -34,13 +34,29 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 
-// The segment and segment-local index of sort block b; false for a surplus
-// block of a capacity-sized grid.  Segments without keys have no blocks (the
-// search takes the last segment starting at or before b).
-__device__ __forceinline__ bool sort_block(const SortSeg* __restrict__ segs, int nseg, uint32_t b,
+// The blocks of a launch are dealt to the 8 XCDs in turn (block i runs on
+// XCD i % 8), each XCD with its own L2.  xcd_block renumbers blocks 0..n-1 so
+// that each XCD takes a contiguous run of sort blocks: the histogram table's
+// lines (one entry per neighbouring block) and the partial lines at the ends
+// of neighbouring blocks' digit runs in the output are then written through
+// one L2, where the pieces merge before they go to HBM.  A bijection of
+// [0, n); the order of the keys does not depend on it.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t n) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t q = n / kXcds, r = n % kXcds, x = i % kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i / kXcds;
+}
+
+// The segment and segment-local index of the sort block that grid block i
+// runs; false for a surplus block of a capacity-sized grid.  Segments
+// without keys have no blocks (the search takes the last segment starting at
+// or before the block).
+__device__ __forceinline__ bool sort_block(const SortSeg* __restrict__ segs, int nseg, uint32_t i,
                                            SortSeg& sg, uint32_t& lb) {
   const SortSeg last = segs[nseg - 1];
-  if (b >= last.block_base + last.nblocks) return false;
+  const uint32_t n = last.block_base + last.nblocks;
+  if (i >= n) return false;
+  const uint32_t b = xcd_block(i, n);
   int lo = 0, hi = nseg - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -122,7 +138,6 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const int lane = t & 63;
   const uint64_t lt = lanemask_lt();
   for (int d = t; d < kDigitBins; d += kSortThreads) {
-    s_glob[d] = offsets[hist_index(sg, (uint32_t)d, lb)];
 #pragma unroll
     for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0u;
   }
@@ -139,6 +154,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     kk[j] = idx < m ? kin[idx] : K(0);
     vv[j] = idx < m ? vin[idx] : 0u;
   }
+  // the block's output offsets (digits t and t + 256), needed only at the
+  // end: loaded after the keys so that their latency hides behind the ranking
+  constexpr int kPerThread = kDigitBins / kSortThreads;
+  uint32_t glob[kPerThread];
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e)
+    glob[e] = offsets[hist_index(sg, (uint32_t)(t + e * kSortThreads), lb)];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
@@ -160,6 +182,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+#pragma unroll
+  for (int e = 0; e < kPerThread; ++e) s_glob[t + e * kSortThreads] = glob[e];
   __syncthreads();
   {  // thread t: digits 2t and 2t + 1, their block-local starts and each wave's start
     constexpr int kPer = kDigitBins / kSortThreads;
