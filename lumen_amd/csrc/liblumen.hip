@@ -274,12 +274,14 @@ __device__ __forceinline__ void splat_box(float i00, float i01, float i11, float
 
 // Pass 2: records and keys of every visible (view, Gaussian) at its scanned
 // offsets, from the geometry k_count cached.  Keys are written by the whole
-// wave together: the wave's keys are one contiguous run (the scan is in
-// Gaussian order), each lane takes every 64th key of the run, finds the
-// splat that owns it by binary search over the lanes' start offsets in LDS,
-// and derives the tile from its rectangle -- coalesced stores and no lane
-// waiting on another lane's long rectangle.  Emission order within a splat
-// is ty-major then tx (pipeline.mojo:152-157).
+// block together: the block's keys are one contiguous run (the scan is in
+// Gaussian order), each thread takes every 256th key of the run, finds the
+// splat that owns it in the block's table of start offsets in LDS, and
+// derives the tile from its rectangle -- coalesced stores, and a splat with
+// thousands of tiles is shared by all 256 threads instead of one wave.  A
+// thread's keys only move forward, so its search continues from the last
+// splat it found (one LDS read while it stays inside a large splat).
+// Emission order within a splat is ty-major then tx (pipeline.mojo:152-157).
 __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __restrict__ views,
                                               int nviews, const uint64_t* __restrict__ offsets,
                                               const uint64_t* __restrict__ totals,
@@ -291,11 +293,15 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
                                               uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals,
                                               SortPlan* __restrict__ plan) {
-  __shared__ uint32_t s_start[4][64], s_rmin[4][64], s_rmax[4][64], s_dk[4][64], s_vi[4][64];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  // per thread of the block: (key start - block base, rect min, rect max,
+  // depth key) and the visible index
+  __shared__ uint4 s_meta[256];
+  __shared__ uint32_t s_vi[256];
+  const int t = threadIdx.x;
+  const uint32_t first = blockIdx.x * 256u;
+  const uint32_t i = first + (uint32_t)t;
   const bool valid = i < s.n;
+  const uint32_t last = min(first + 256u, s.n) - 1u;  // the block's last Gaussian
   const uint64_t nv = (uint64_t)nviews * s.n;
   float px = 0.0f, py = 0.0f, pz = 0.0f, op = 0.0f;
   if (valid) {
@@ -314,25 +320,27 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
   float c[48];
   for (int vw = 0; vw < nviews; ++vw) {
     const ViewParams& v = views[vw];
-    uint32_t vis = 0, cnt = 0, vi = 0, ko = 0;
+    // the block's runs: keys [base, end), visible splats [vbase, vend)
+    const uint64_t row = (uint64_t)vw * s.n;
+    const uint64_t o_first = offsets[row + first];
+    const uint64_t o_end = row + last + 1 < nv ? offsets[row + last + 1] : *totals;
+    const uint32_t base = (uint32_t)(o_first >> 32), end = (uint32_t)(o_end >> 32);
+    const uint32_t vend = (uint32_t)o_end;
+    if ((uint64_t)end > m_cap || (uint64_t)vend > v_cap) {  // uniform across the block
+      if (t == 0) atomicOr(&plan->overflow, 1u);
+      continue;
+    }
+    uint32_t vis = 0, ko = base, vi = 0;
     if (valid) {
-      const uint64_t idx = (uint64_t)vw * s.n + i;
-      const uint64_t off0 = offsets[idx];
-      const uint64_t off1 = idx + 1 < nv ? offsets[idx + 1] : *totals;
+      const uint64_t off0 = offsets[row + i];
+      const uint64_t off1 = row + i + 1 < nv ? offsets[row + i + 1] : *totals;
       vis = (uint32_t)off1 - (uint32_t)off0;
-      cnt = (uint32_t)(off1 >> 32) - (uint32_t)(off0 >> 32);
       vi = (uint32_t)off0;
       ko = (uint32_t)(off0 >> 32);
     }
-    // The wave's run of keys is [base, end); lane 0 is valid whenever any lane is.
-    const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)ko);
-    const uint32_t end = wave_max_u32_all(valid ? ko + cnt : 0u);
-    const uint32_t vend = wave_max_u32_all(vis ? vi + 1u : 0u);
-    const bool overflow = (uint64_t)end > m_cap || (uint64_t)vend > v_cap;
-    if (overflow && lane == 0) atomicOr(&plan->overflow, 1u);
     uint32_t rmin = 0, rmax = 0, dk = 0;
-    if (vis && !overflow) {
-      const GeoCache gc = geo[(uint64_t)vw * s.n + i];
+    if (vis) {
+      const GeoCache gc = geo[row + i];
       const float sx = gc.a.x, sy = gc.a.y, depth = gc.a.z;
       const float c00 = gc.a.w, c01 = gc.b.x, c11 = gc.b.y;
       rmin = __float_as_uint(gc.b.z);
@@ -379,40 +387,39 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       vmeta[vi] = make_uint4(ko, rmin, rmax, 0u);
       dk = depth_key(depth);
     }
-    if (overflow) continue;  // uniform across the wave
-    s_start[w][lane] = valid ? ko - base : 0xFFFFFFFFu;
-    s_rmin[w][lane] = rmin;
-    s_rmax[w][lane] = rmax;
-    s_dk[w][lane] = dk;
-    s_vi[w][lane] = vi;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // threads past the scene never own a key; one with no key shares its
+    // start with the next splat that has keys, and the search takes the last
+    // of equal starts
+    s_meta[t] = make_uint4(valid ? ko - base : 0xFFFFFFFFu, rmin, rmax, dk);
+    s_vi[t] = vi;
+    __syncthreads();
     const uint32_t total = end - base;
-    for (uint32_t p = lane; p < total; p += 64) {
-      int lo = 0, hi = 63;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_start[w][mid] <= p) lo = mid;
-        else hi = mid - 1;
+    int g = 0;  // the splat owning key p (thread 0 starts at 0)
+    for (uint32_t p = (uint32_t)t; p < total; p += 256u) {
+      if (g < 255 && s_meta[g + 1].x <= p) {
+        int lo = g + 1, hi = 255;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_meta[mid].x <= p) lo = mid;
+          else hi = mid - 1;
+        }
+        g = lo;
       }
-      const uint32_t local = p - s_start[w][lo];
-      const uint32_t r0 = s_rmin[w][lo], r1 = s_rmax[w][lo];
-      const uint32_t min_tx = r0 & 0xFFFFu, min_ty = r0 >> 16;
-      const uint32_t wtiles = (r1 & 0xFFFFu) - min_tx + 1u;
-      const uint32_t row = local / wtiles;
-      const uint32_t ty = min_ty + row;
-      const uint32_t tx = min_tx + (local - row * wtiles);
+      const uint4 m = s_meta[g];
+      const uint32_t local = p - m.x;
+      const uint32_t min_tx = m.y & 0xFFFFu, min_ty = m.y >> 16;
+      const uint32_t wtiles = (m.z & 0xFFFFu) - min_tx + 1u;
+      const uint32_t r = local / wtiles;
+      const uint32_t ty = min_ty + r;
+      const uint32_t tx = min_tx + (local - r * wtiles);
       const uint32_t tile = ty * (uint32_t)v.tiles_x + tx;  // within the view
       if (frame_keys)
-        keys32[base + p] = ((tile - kt0) << kdb) | (s_dk[w][lo] - kd0);
+        keys32[base + p] = ((tile - kt0) << kdb) | (m.w - kd0);
       else
-        keys[base + p] = ((uint64_t)tile << 32) | (uint64_t)s_dk[w][lo];
-      vals[base + p] = s_vi[w][lo];
+        keys[base + p] = ((uint64_t)tile << 32) | (uint64_t)m.w;
+      vals[base + p] = s_vi[g];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
   }
 }
 
