@@ -442,45 +442,62 @@ __device__ __forceinline__ uint64_t sorted_key(const void* keys, uint32_t i, boo
 // Tile ranges from the sorted keys.  A run of one tile starts where the
 // view-local tile changes or a view's segment begins; the thread at a run's
 // start also closes the previous run, and the last key closes the final
-// one.  The previous key comes from the lane below (lane 0 loads it).
+// one.  A block takes 2048 consecutive keys, 8 per thread (all loads issued
+// before any use), and finds each key's predecessor in an LDS copy of the
+// block's keys; thread 0 loads the key before the block.
+constexpr int kRangeItems = 8;
+constexpr uint32_t kRangeKeys = 256u * kRangeItems;
+
 __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
                                                 const uint64_t* __restrict__ k1,
                                                 const SortPlan* __restrict__ plan,
                                                 const SortSeg* __restrict__ segs, int nseg,
                                                 const ViewParams* __restrict__ views,
                                                 int2* __restrict__ ranges, uint32_t tiles) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  __shared__ uint64_t s_key[kRangeKeys + 1];  // s_key[1 + j]: the block's key j
   const uint32_t m = plan->m;
-  if (blockIdx.x * 256u >= m || plan->overflow) return;  // uniform across the block
-  const bool valid = i < m;
+  const uint32_t b0 = blockIdx.x * kRangeKeys;
+  if (b0 >= m || plan->overflow) return;  // uniform across the block
   const void* keys = (plan->passes & 1) ? (const void*)k1 : (const void*)k0;
   const SortPlan p = *plan;
   const bool fk = p.mode == 1;
-  const uint64_t k = valid ? sorted_key(keys, i, fk) : 0ull;
-  uint64_t kp = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(k >> 32), 1, 64) << 32) |
-                (uint32_t)__shfl_up((int)(uint32_t)k, 1, 64);
-  if ((threadIdx.x & 63) == 0 && valid && i > 0) kp = sorted_key(keys, i - 1, fk);
-  if (!valid) return;
-  // the segment of the block's first key (a search on uniform values), then
-  // forward to this key's
-  int sv = seg_of(segs, nseg, blockIdx.x * 256u);
-  while (sv + 1 < nseg && segs[sv + 1].key_base <= i) ++sv;
-  const bool seg_start = i == segs[sv].key_base;
-  const uint32_t tile = frame_key_tile(k, p);
-  const uint32_t gt = views[sv].tile_base + tile;
-  if (i == 0 || seg_start || frame_key_tile(kp, p) != tile) {
-    if (gt < tiles) ranges[gt].x = (int32_t)i;
-    if (i > 0) {  // close the previous run, in its own view
-      int sp = sv;
-      if (seg_start) {
-        sp = sv - 1;
-        while (sp > 0 && segs[sp].count == 0) --sp;
-      }
-      const uint32_t gp = views[sp].tile_base + frame_key_tile(kp, p);
-      if (gp < tiles) ranges[gp].y = (int32_t)i;
-    }
+  const int t = threadIdx.x;
+  uint64_t k[kRangeItems];
+#pragma unroll
+  for (int j = 0; j < kRangeItems; ++j) {
+    const uint32_t i = b0 + (uint32_t)j * 256u + (uint32_t)t;
+    k[j] = i < m ? sorted_key(keys, i, fk) : 0ull;
   }
-  if (i + 1 == m && gt < tiles) ranges[gt].y = (int32_t)m;
+  if (t == 0) s_key[0] = b0 > 0 ? sorted_key(keys, b0 - 1, fk) : 0ull;
+#pragma unroll
+  for (int j = 0; j < kRangeItems; ++j) s_key[1 + j * 256 + t] = k[j];
+  __syncthreads();
+  // the segment of the block's first key (a search on uniform values), then
+  // forward with the thread's keys
+  int sv = seg_of(segs, nseg, b0);
+#pragma unroll
+  for (int j = 0; j < kRangeItems; ++j) {
+    const uint32_t i = b0 + (uint32_t)j * 256u + (uint32_t)t;
+    if (i >= m) break;
+    const uint64_t kp = s_key[j * 256 + t];  // key i - 1
+    while (sv + 1 < nseg && segs[sv + 1].key_base <= i) ++sv;
+    const bool seg_start = i == segs[sv].key_base;
+    const uint32_t tile = frame_key_tile(k[j], p);
+    const uint32_t gt = views[sv].tile_base + tile;
+    if (i == 0 || seg_start || frame_key_tile(kp, p) != tile) {
+      if (gt < tiles) ranges[gt].x = (int32_t)i;
+      if (i > 0) {  // close the previous run, in its own view
+        int sp = sv;
+        if (seg_start) {
+          sp = sv - 1;
+          while (sp > 0 && segs[sp].count == 0) --sp;
+        }
+        const uint32_t gp = views[sp].tile_base + frame_key_tile(kp, p);
+        if (gp < tiles) ranges[gp].y = (int32_t)i;
+      }
+    }
+    if (i + 1 == m && gt < tiles) ranges[gt].y = (int32_t)m;
+  }
 }
 
 // Frame keys (either width) back to the reference's 64-bit keys, for
@@ -948,7 +965,7 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                                  SortScratch{&c->hist, &c->hist_part}, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
-    hipLaunchKernelGGL(k_ranges, dim3(div_up(m_cap, 256)), dim3(256), 0, st, kb[0], kb[1], plan,
+    hipLaunchKernelGGL(k_ranges, dim3(div_up(m_cap, kRangeKeys)), dim3(256), 0, st, kb[0], kb[1], plan,
                        c->segs.as<SortSeg>(), (int)fs.nviews, c->views.as<ViewParams>(),
                        c->ranges.as<int2>(), (uint32_t)fs.tiles);
     LUMEN_LAUNCHED();

@@ -101,10 +101,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
   __syncthreads();
   const uint32_t base = sg.key_base + lb * kSortTile;
   const uint32_t end = sg.key_base + sg.count;
-#pragma unroll 4
+  // all loads first (a load feeding an atomic in the same step would wait
+  // for each key in turn)
+  K kk[kSortItems];
+#pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const uint32_t idx = base + j * kSortThreads + t;
-    if (idx < end) atomicAdd(&h[w][(uint32_t)(keys[idx] >> shift) & mask], 1u);
+    kk[j] = idx < end ? keys[idx] : K(0);
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    if (base + j * kSortThreads + t < end)
+      atomicAdd(&h[w][(uint32_t)(kk[j] >> shift) & mask], 1u);
   }
   __syncthreads();
   for (int d = t; d < kDigitBins; d += kSortThreads)
@@ -117,9 +125,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const SortSeg* __restrict__ segs, int nseg, int pass, const uint32_t* __restrict__ offsets) {
   __shared__ K s_key[kSortTile];
   __shared__ uint32_t s_val[kSortTile];
-  __shared__ uint32_t s_cnt[4][kDigitBins];  // per-wave digit counts, then per-wave starts
-  __shared__ uint32_t s_start[kDigitBins];   // block-local start of each digit
-  __shared__ uint32_t s_glob[kDigitBins];    // global start of each digit for this block
+  // per-wave digit counts, then per-wave starts (both <= kSortTile; 16 bits
+  // keep the block's LDS under 40 KB with 32-bit keys: four blocks per CU)
+  __shared__ uint16_t s_cnt[4][kDigitBins];
+  // output position of the block's key at LDS slot i of digit d: s_delta[d] + i
+  // (the digit's global start minus its block-local start, modulo 2^32)
+  __shared__ uint32_t s_delta[kDigitBins];
   __shared__ uint32_t s_w[4];
   if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
   SortSeg sg;
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const uint64_t lt = lanemask_lt();
   for (int d = t; d < kDigitBins; d += kSortThreads) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0u;
+    for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0;
   }
   __syncthreads();
   // wave w owns keys [base + 1024 w, base + 1024 (w + 1)), 64 per round
@@ -154,13 +165,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     kk[j] = idx < m ? kin[idx] : K(0);
     vv[j] = idx < m ? vin[idx] : 0u;
   }
-  // the block's output offsets (digits t and t + 256), needed only at the
+  // the block's output offsets (digits 2t and 2t + 1), needed only at the
   // end: loaded after the keys so that their latency hides behind the ranking
-  constexpr int kPerThread = kDigitBins / kSortThreads;
-  uint32_t glob[kPerThread];
+  constexpr int kPer = kDigitBins / kSortThreads;
+  uint32_t glob[kPer];
 #pragma unroll
-  for (int e = 0; e < kPerThread; ++e)
-    glob[e] = offsets[hist_index(sg, (uint32_t)(t + e * kSortThreads), lb)];
+  for (int e = 0; e < kPer; ++e) glob[e] = offsets[hist_index(sg, (uint32_t)(kPer * t + e), lb)];
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
@@ -177,16 +187,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (valid && rank == 0) s_cnt[w][d] = before + (uint32_t)__popcll(peers);
+    if (valid && rank == 0) s_cnt[w][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-#pragma unroll
-  for (int e = 0; e < kPerThread; ++e) s_glob[t + e * kSortThreads] = glob[e];
   __syncthreads();
   {  // thread t: digits 2t and 2t + 1, their block-local starts and each wave's start
-    constexpr int kPer = kDigitBins / kSortThreads;
     uint32_t c[kPer][4], sum = 0;
 #pragma unroll
     for (int e = 0; e < kPer; ++e)
@@ -200,10 +207,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
       const int d = kPer * t + e;
-      s_start[d] = st;
+      s_delta[d] = glob[e] - st;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        s_cnt[i][d] = st;
+        s_cnt[i][d] = (uint16_t)st;
         st += c[e][i];
       }
     }
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   for (uint32_t i = t; i < nblk; i += kSortThreads) {
     const K key = s_key[i];
     const uint32_t d = (uint32_t)(key >> shift) & mask;
-    const uint32_t dst = s_glob[d] + (i - s_start[d]);
+    const uint32_t dst = s_delta[d] + i;
     kout[dst] = key;
     vout[dst] = s_val[i];
   }
