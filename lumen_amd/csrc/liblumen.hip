@@ -603,8 +603,12 @@ __global__ __launch_bounds__(256) void k_composite(
         const float lim = power_limit(b.y);
         float hx = c.z, hy = c.w;
         limit_box(lim, hx, hy);
-        s0[j] = a;                                // x, y, i00, 2 i01
-        s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
+        // the power's factor 0.5 goes into the conic.  Halving commutes with
+        // each rounding of 0.5 (i00 dx dx + 2 i01 dx dy + i11 dy dy) while the
+        // terms are normal floats, so the power keeps its bits; a power small
+        // enough for that to fail (< 2^-125) gives exp(-power) = 1 either way
+        s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
+        s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
         s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
         s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, tx * v.tile_w, ty * v.tile_h,
                                         v.tile_w, rows, nwaves);
@@ -625,8 +629,8 @@ __global__ __launch_bounds__(256) void k_composite(
       if (any) {
         for (uint32_t q = 0; q < cnt; ++q) {
           const int j = s_list[w][q];
-          const float4 a = s0[j];  // x, y, i00, 2 i01
-          const float4 b = s1[j];  // i11, opacity, power limit
+          const float4 a = s0[j];  // x, y, i00/2, i01
+          const float4 b = s1[j];  // i11/2, opacity, power limit
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
           const float txx = a.z * dx * dx;
@@ -635,7 +639,7 @@ __global__ __launch_bounds__(256) void k_composite(
           for (int p = 0; p < PIX; ++p) {
             if (!live[p]) continue;
             const float dy = pyc[p] - a.y;
-            float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
+            float power = txx + txy * dy + b.x * dy * dy;
             if (power < 0.0f) power = 0.0f;
             if (power > b.z) continue;  // power > 4.5, or alpha < 1/255 (power_limit)
             float alpha = b.y * splat_exp_neg(-power);
