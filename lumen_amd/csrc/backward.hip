@@ -52,16 +52,12 @@ __device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ vie
   return lo;
 }
 
-// Zeroes the frame's instance slots (a slot the reverse pass never reaches
-// stays zero) and records, for every kSumChunk-slot chunk, the visible splat
-// whose slots contain the chunk's first slot.
-__global__ void k_zero_partial(float* __restrict__ g, const SortPlan* __restrict__ plan,
-                               const uint4* __restrict__ vmeta,
-                               uint32_t* __restrict__ chunk_seg) {
+// Records, for every kSumChunk-slot chunk of the instance slots, the visible
+// splat whose slots contain the chunk's first slot.
+__global__ void k_chunk_seg(const SortPlan* __restrict__ plan, const uint4* __restrict__ vmeta,
+                            uint32_t* __restrict__ chunk_seg) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n = (uint64_t)plan->m * kGradVis;
-  for (uint64_t i = i0; i < n; i += stride) g[i] = 0.0f;
   const uint32_t V = plan->v, M = plan->m;
   for (uint64_t k = i0; k < V; k += stride) {
     const uint32_t q0 = vmeta[k].x;
@@ -140,6 +136,17 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   __syncthreads();
   const int block_last = s_last;
   const float pxc = (float)px + 0.5f;
+  // the slots of the instances past every pixel's last evaluated splat are
+  // zero (the reverse pass never reaches them; it writes every other slot)
+  for (int q = block_last + 1 + t; q < rg.y; q += nthreads) {
+    const uint4 mt = vmeta[vals[q]];
+    const uint32_t min_tx = mt.y & 0xFFFFu, min_ty = mt.y >> 16;
+    const uint32_t wtiles = (mt.z & 0xFFFFu) - min_tx + 1u;
+    const uint32_t dst = mt.x + ((uint32_t)ty - min_ty) * wtiles + ((uint32_t)tx - min_tx);
+    float2* d2 = reinterpret_cast<float2*>(partial + (size_t)dst * kGradVis);
+#pragma unroll
+    for (int c = 0; c < kGradVis / 2; ++c) d2[c] = make_float2(0.0f, 0.0f);
+  }
   for (int end = block_last + 1; end > rg.x; end -= BATCH) {
     const int start = max(rg.x, end - BATCH);
     const int n = end - start;
@@ -257,7 +264,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         x += s_part[w][e];
         s_part[w][e] = 0.0f;
       }
-      if (x != 0.0f) partial[(size_t)s_dst[e / kGradVis] * kGradVis + e % kGradVis] = x;
+      partial[(size_t)s_dst[e / kGradVis] * kGradVis + e % kGradVis] = x;
     }
     __syncthreads();
   }
@@ -738,8 +745,8 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   const SortPlan* plan = c->plan.as<SortPlan>();
   float* part = c->partial.as<float>();
   const uint4* vmeta = c->vmeta.as<uint4>();
-  const uint32_t zb = div_up(m_cap * kGradVis, 1024) < 8192 ? div_up(m_cap * kGradVis, 1024) : 8192;
-  hipLaunchKernelGGL(k_zero_partial, dim3(zb ? zb : 1), dim3(256), 0, st, part, plan, vmeta,
+  const uint32_t zb = div_up(v_cap, 256) < 8192 ? div_up(v_cap, 256) : 8192;
+  hipLaunchKernelGGL(k_chunk_seg, dim3(zb ? zb : 1), dim3(256), 0, st, plan, vmeta,
                      c->sum_seg.as<uint32_t>());
   LUMEN_LAUNCHED();
   const uint32_t threads = cfg->tile_width * cfg->tile_height;
