@@ -133,6 +133,8 @@ struct LumenExContext {
   bool events_ready = false;
   bool stage_ran[LUMEN_EX_TIME_STAGES] = {};
   hipEvent_t ev[2 * LUMEN_EX_TIME_STAGES] = {};
+  // the plain frame waits on this event (recorded after the totals' copy)
+  hipEvent_t ev_totals = nullptr;
 };
 
 // Records the start (end = false) or end of a timed stage on the stream when

@@ -28,6 +28,8 @@
 // (to size the key buffers); the graph path (LUMEN_EX_GRAPH) captures the
 // whole frame once with reserved capacities and replays it.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdio.h>
 #include <string.h>
 
@@ -925,12 +927,11 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
   return LUMEN_SUCCESS;
 }
 
-// Stage 2 of a frame: plan, emission, sort, ranges and compositing, with the
-// given capacities (exact after a host read-back, reserved in a graph).
-static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
-                                const LumenExRenderConfig* cfg, uint64_t v_cap, uint64_t m_cap,
-                                int sort_slots, int sort_modes, bool keep, float* out_rgb,
-                                float* out_depth, hipStream_t st) {
+// Stage 2 of a frame: plan and emission, with the given capacities (exact
+// after a host read-back, reserved in a graph, or the buffers' present size
+// ahead of the read-back; k_plan and k_emit flag a frame that exceeds them).
+static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
+                                uint64_t v_cap, uint64_t m_cap, hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
     const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 256 * 16);
@@ -953,6 +954,17 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                        c->vals[0].as<uint32_t>(), plan);
     LUMEN_LAUNCHED();
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, true, st));
+  }
+  return LUMEN_SUCCESS;
+}
+
+// Stage 3 of a frame: sort, ranges and compositing of at most m_cap keys.
+static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
+                                  const LumenExRenderConfig* cfg, uint64_t m_cap, int sort_slots,
+                                  int sort_modes, bool keep, float* out_rgb, float* out_depth,
+                                  hipStream_t st) {
+  SortPlan* plan = c->plan.as<SortPlan>();
+  if (sd.n > 0 && m_cap > 0) {
     if (keep) {
       LUMEN_TRY(ensure(c->keys_keep, m_cap * 8));
       LUMEN_TRY(ensure(c->vals_keep, m_cap * 4));
@@ -989,6 +1001,30 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
   return LUMEN_SUCCESS;
 }
 
+// Stages 2 and 3 with one set of capacities (the graph frame).
+static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
+                                const LumenExRenderConfig* cfg, uint64_t v_cap, uint64_t m_cap,
+                                int sort_slots, int sort_modes, bool keep, float* out_rgb,
+                                float* out_depth, hipStream_t st) {
+  LUMEN_TRY(enqueue_emit(c, sd, fs, v_cap, m_cap, st));
+  return enqueue_finish(c, sd, fs, cfg, m_cap, sort_slots, sort_modes, keep, out_rgb, out_depth,
+                        st);
+}
+
+// The visible-splat and key capacities the buffers hold now (0 before a
+// frame has sized them).
+static void current_capacity(const LumenExContext* c, uint64_t& v_cap, uint64_t& m_cap) {
+  auto cap = [](const DevBuf& b, uint64_t unit) -> uint64_t { return b.ptr ? b.cap / unit : 0; };
+  v_cap = cap(c->rec, 48);
+  v_cap = std::min(v_cap, cap(c->vis2orig, 4));
+  v_cap = std::min(v_cap, cap(c->radius, 4));
+  v_cap = std::min(v_cap, cap(c->vmeta, 16));
+  m_cap = cap(c->keys[0], 8);
+  m_cap = std::min(m_cap, cap(c->keys[1], 8));
+  m_cap = std::min(m_cap, cap(c->vals[0], 4));
+  m_cap = std::min(m_cap, cap(c->vals[1], 4));
+}
+
 // Sort passes that can ever be needed for this frame shape: depth keys are
 // at most trunc(far * 1000) (culling keeps depth <= far), frame tile ids at
 // most tiles - 1.
@@ -1022,28 +1058,48 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   LUMEN_TRY(enqueue_count(c, sd, fs, st));
   uint64_t V = 0, M = 0;
   uint32_t hb[4] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u};
+  uint64_t v_now = 0, m_now = 0;
+  bool ahead = false;  // plan and emission enqueued before the read-back
   if (sd.n > 0) {
     uint8_t* h = static_cast<uint8_t*>(c->pinned);
     LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 24, hipMemcpyDeviceToHost, st));  // totals, bounds
-    LUMEN_HIP(hipStreamSynchronize(st));
+    if (!c->ev_totals) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_totals, hipEventDisableTiming));
+    LUMEN_HIP(hipEventRecord(c->ev_totals, st));
+    // Plan and emission go ahead with the capacity the buffers already hold,
+    // so the device works on them while the host waits for the totals (an
+    // event on the copy, not the stream); a frame that outgrows them is
+    // flagged by k_plan / k_emit and both run again below.
+    current_capacity(c, v_now, m_now);
+    if (v_now > 0 && m_now > 0) {
+      LUMEN_TRY(enqueue_emit(c, sd, fs, v_now, m_now, st));
+      ahead = true;
+    }
+    LUMEN_HIP(hipEventSynchronize(c->ev_totals));
     uint64_t tot;
     memcpy(&tot, h, 8);
     memcpy(hb, h + 8, 16);
     V = tot & 0xFFFFFFFFull;
     M = tot >> 32;
-    if (M >= 0x7FFFFFFFull)
+    if (M >= 0x7FFFFFFFull) {
+      if (ahead) LUMEN_HIP(hipStreamSynchronize(st));
       return fail(LUMEN_EX_ERROR_CAPACITY, "%llu tile instances exceed 2^31-1",
                   (unsigned long long)M);
+    }
   }
-  LUMEN_TRY(ensure_capacity(c, V, M));
+  if (!(ahead && V <= v_now && M <= m_now)) {
+    // the buffers may be replaced: the emission enqueued above must be done
+    if (ahead) LUMEN_HIP(hipStreamSynchronize(st));
+    LUMEN_TRY(ensure_capacity(c, V, M));
+    LUMEN_TRY(enqueue_emit(c, sd, fs, V, M, st));
+  }
   // the plan k_plan will write, known here from the bounds: only its passes
   // and key width are enqueued
   SortPlan hp;
   plan_digits(V > 0, hb[0], hb[1], hb[2], hb[3], hp);
   const int np = hp.passes;
   const bool keep = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0 && M > 0;
-  LUMEN_TRY(enqueue_rest(c, sd, fs, cfg, V, M, np, hp.mode == 1 ? 2 : 1, keep, out_rgb,
-                         out_depth, st));
+  LUMEN_TRY(enqueue_finish(c, sd, fs, cfg, M, np, hp.mode == 1 ? 2 : 1, keep, out_rgb,
+                           out_depth, st));
   c->V = V;
   c->M = M;
   c->T = fs.tiles;
@@ -1270,6 +1326,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   graph_release(c->graph);
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->ev_totals) (void)hipEventDestroy(c->ev_totals);
   for (DevBuf* b : context_buffers(c)) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);
