@@ -43,8 +43,10 @@ static int64_t f2i_trunc(float f) {
 }
 
 /* exp(x): Cody-Waite reduction x = k*ln2 + r (ln2 split in a 15-bit head and
- * a tail), degree-7 Taylor polynomial of e^r on |r| <= ln2/2 in Horner form
- * with fused multiply-adds, then scaling by 2^k.  Every step is an IEEE
+ * a tail), a degree-6 polynomial of e^r on |r| <= ln2/2 in Horner form with
+ * fused multiply-adds (constant term 1, the others a minimax fit of the
+ * relative error rounded to float32: oracle/exp_poly_fit.py), then scaling
+ * by 2^k.  Every step is an IEEE
  * operation with a single defined rounding (rint, fma, ldexp), so this
  * function returns identical bits on any IEEE machine; the HIP compositor
  * implements the same steps.  Max error vs exp in float64 is checked in
@@ -55,12 +57,11 @@ float orc_expf(float x) {
     float k = rintf(x * 1.44269504f);
     float r = fmaf(k, -0.693145751953125f, x);
     r = fmaf(k, -1.42860677e-06f, r);
-    float p = 1.98412698e-4f;
-    p = fmaf(p, r, 1.38888889e-3f);
-    p = fmaf(p, r, 8.33333333e-3f);
-    p = fmaf(p, r, 4.16666667e-2f);
-    p = fmaf(p, r, 1.66666667e-1f);
-    p = fmaf(p, r, 0.5f);
+    float p = 1.384365372e-3f;
+    p = fmaf(p, r, 8.374155499e-3f);
+    p = fmaf(p, r, 4.166800156e-2f);
+    p = fmaf(p, r, 1.666643173e-1f);
+    p = fmaf(p, r, 4.999999404e-1f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     return ldexpf(p, (int)k);

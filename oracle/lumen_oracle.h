@@ -40,8 +40,8 @@ typedef struct {
     int64_t max_sh_degree;
     int32_t white_background;
     int32_t compute_depth;
-    /* 0: exp from orc_expf (the Cody-Waite + Taylor-7 exp the HIP kernel
-     *    also implements); 1: libm expf.  Mojo's std.math.exp is neither;
+    /* 0: exp from orc_expf (the Cody-Waite + degree-6 polynomial exp the HIP
+     *    kernel also implements); 1: libm expf.  Mojo's std.math.exp is neither;
      *    both agree with it to ~1 ulp. */
     int32_t exp_mode;
 } orc_config;

@@ -78,6 +78,25 @@ def test_random100k_1280x720(rasterizer, oracle):
     assert ref["instances"] > 400_000
 
 
+def test_capacity_growth_and_reuse(oracle):
+    """A fresh context renders a small cloud, a larger one and the small one
+    again.  The plain frame enqueues plan and emission with the capacity its
+    buffers already hold before it reads the totals back: the larger frame
+    overflows that capacity (flagged on the device, both kernels run again
+    after the buffers grow), the third fits inside it.  Every frame is the
+    oracle's, stage by stage."""
+    from lumen_amd.raster import Rasterizer
+    cam = look_at(CameraConfig(640, 480, 600.0, 600.0, 320.0, 240.0, 0.01, 100.0), (0.0, 0.0, 5.0))
+    small = random_scene(3_000, seed=2101)
+    big = random_scene(40_000, seed=2102)
+    R = Rasterizer(0)
+    counts = []
+    for sc in (small, big, small, big):
+        ref, out = compare(R, oracle, sc, cam)
+        counts.append(ref["instances"])
+    assert counts[1] > 4 * counts[0] > 0
+
+
 @pytest.mark.parametrize("tw,th,batch", [(8, 8, 64), (32, 8, 256), (16, 16, 64), (16, 16, 128),
                                          (16, 8, 128), (8, 16, 256)])
 def test_tile_shapes_and_batches(rasterizer, oracle, tw, th, batch):
