@@ -158,6 +158,8 @@ class RasterizeViews(torch.autograd.Function):
     @staticmethod
     def forward(ctx, planes, sh, cameras, config, renderer: MultiViewRenderer, graph: bool,
                 viewspace):
+        # an unused output's gradient arrives as None, not as a zero-filled image
+        ctx.set_materialize_grads(False)
         scene = DeviceScene(planes.detach(), sh.detach())
         if graph:
             # replays need fixed output pointers: the renderer's planes are

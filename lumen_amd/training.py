@@ -109,6 +109,9 @@ class RasterizeSplats(torch.autograd.Function):
     @staticmethod
     def forward(ctx, planes, sh, camera: PinholeCamera, config: RenderConfig,
                 rasterizer: Rasterizer, viewspace: Optional[GaussianGradients]):
+        # an unused output's gradient (the depth's, usually) arrives as None,
+        # not as a zero-filled image
+        ctx.set_materialize_grads(False)
         scene = DeviceScene(planes.detach(), sh.detach())
         out = rasterizer.render(scene, camera, config, save_pixels=True)
         ctx.scene = scene
@@ -175,13 +178,16 @@ class PhotometricLoss(torch.autograd.Function):
         _lib.check("lumen_ex_photometric_loss", st)
         ctx.save_for_backward(grad if grad is not None else out)
         ctx.has_grad = grad is not None
-        l1, ssim = out[0].clone(), out[1].clone()
+        # the three values are views of `out` (no copies); the gradients of
+        # L1 and SSIM are never needed, so they are not materialized either
+        ctx.set_materialize_grads(False)
+        l1, ssim = out[0], out[1]
         ctx.mark_non_differentiable(l1, ssim)
-        return out[2].clone(), l1, ssim
+        return out[2], l1, ssim
 
     @staticmethod
     def backward(ctx, g_loss, g_l1, g_ssim):
-        if not ctx.has_grad:
+        if not ctx.has_grad or g_loss is None:
             return None, None, None, None
         (grad,) = ctx.saved_tensors
         return grad * g_loss, None, None, None
