@@ -241,6 +241,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
       segs[vw].block_base = b;
       b += segs[vw].nblocks;
     }
+    plan->blocks = b;
   }
 }
 
@@ -714,6 +715,7 @@ __global__ void k_set_plan(SortPlan* plan, SortSeg* seg, int passes, const int32
   seg->count = m;
   seg->block_base = 0;
   seg->nblocks = (m + kSortTile - 1) / kSortTile;
+  plan->blocks = seg->nblocks;
   plan->passes = passes;
   for (int k = 0; k < 8; ++k) {
     plan->shift[k] = k < passes ? 8 * digits[k] : 0;

@@ -31,7 +31,7 @@ struct SortPlan {
   int32_t mode;        // 0: 64-bit keys, 1: 32-bit frame keys
   uint32_t tmin, dmin; // frame-key offsets (mode 1)
   int32_t dbits;       // depth bits of a frame key (mode 1)
-  uint32_t pad;
+  uint32_t blocks;     // sort blocks over all segments (the histogram table's rows / 512)
 };
 
 // One segment of a frame's keys: a view's contiguous run (emission is

@@ -18,13 +18,23 @@ constexpr int kScanChunk = kScanThreads * kScanItems;
 __device__ __forceinline__ int scan_pad(int i) { return i + (i >> 5); }
 
 // A scan that runs only when *value > above (value null: always), so that a
-// graph can hold scans of sort passes the frame's device plan may not use.
+// graph can hold scans of sort passes the frame's device plan may not use,
+// over at most *len * len_mul elements (len null: the launch's n), so that a
+// graph sized for a capacity scans only what the frame filled.
 struct ScanGate {
   const int32_t* value = nullptr;
   int32_t above = 0;
+  const uint32_t* len = nullptr;
+  uint32_t len_mul = 1;
 };
 
 __device__ __forceinline__ bool gate_closed(ScanGate g) { return g.value && !(*g.value > g.above); }
+
+__device__ __forceinline__ uint64_t gate_len(ScanGate g, uint64_t n) {
+  if (!g.len) return n;
+  const uint64_t m = (uint64_t)*g.len * g.len_mul;
+  return m < n ? m : n;
+}
 
 template <typename T>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restrict__ in, uint64_t n,
@@ -32,6 +42,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const T* __restric
   if (gate_closed(gate)) return;
   __shared__ T s_w[kScanThreads / 64];
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  n = gate_len(gate, n);
+  if (base >= n) {  // past the filled length: an empty chunk
+    if (threadIdx.x == 0) partials[blockIdx.x] = T(0);
+    return;
+  }
   T s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
@@ -74,6 +89,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict_
   __shared__ T s[kScanChunk + kScanChunk / 32];
   __shared__ T s_w[kScanThreads / 64];
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  n = gate_len(gate, n);
+  if (base >= n) return;  // past the filled length: nothing to write
   const int t = threadIdx.x;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
@@ -121,6 +138,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
   __shared__ T s_w[kScanThreads / 64];
   __shared__ T s_prefix;
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  n = gate_len(gate, n);
+  // past the filled length nothing is written, except the total by the last block
+  if (base >= n && !(total_out && blockIdx.x == gridDim.x - 1)) return;
   const int t = threadIdx.x;
   {
     T p = 0;

@@ -283,9 +283,10 @@ inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const Sort
                          k32[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
-    // the pass's scan runs only when the device plan has the pass
+    // the pass's scan runs only when the device plan has the pass, and over
+    // the table rows of the frame's sort blocks (a graph's grid covers its capacity)
     LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st,
-                                       ScanGate{&plan->passes, p}));
+                                       ScanGate{&plan->passes, p, &plan->blocks, kDigitBins}));
     if (modes & 1) {
       hipLaunchKernelGGL(k_sort_scatter<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0],
                          k[1], v[0], v[1], plan, segs, nseg, p, hist);
