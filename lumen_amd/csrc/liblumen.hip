@@ -192,13 +192,17 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
                        const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
                        uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
                        ViewSpan* __restrict__ spans, SortSeg* __restrict__ segs,
-                       int2* __restrict__ ranges, uint32_t tiles) {
+                       int2* __restrict__ ranges, uint32_t tiles, int force_items) {
   // every block: empty tile ranges (-1, -1) before k_ranges fills the used ones
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
     ranges[t] = make_int2(-1, -1);
   if (blockIdx.x != 0) return;
   const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
+  // keys per thread of the sort blocks: a graph fixes it, a plain frame takes
+  // it from its key count (the host decides the same from the read-back)
+  const int items = force_items ? force_items : sort_items_for(M);
+  const uint32_t tile = 256u * (uint32_t)items;
   if (threadIdx.x == 0) {
     SortPlan p;
     plan_digits(V > 0, bounds[0], bounds[1], bounds[2], bounds[3], p);
@@ -214,6 +218,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     plan->m = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
     plan->overflow = ((uint64_t)M > m_cap || (uint64_t)V > v_cap) ? 1u : 0u;
+    plan->items = items;
   }
   const uint32_t mc = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
   for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
@@ -230,7 +235,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     sg.key_base = sp.key_base < mc ? sp.key_base : mc;
     const uint32_t kend = sp.key_base + sp.instances < mc ? sp.key_base + sp.instances : mc;
     sg.count = kend - sg.key_base;
-    sg.nblocks = (sg.count + kSortTile - 1) / kSortTile;
+    sg.nblocks = (sg.count + tile - 1) / tile;
     sg.block_base = 0;
     segs[vw] = sg;
   }
@@ -710,12 +715,13 @@ __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
 }
 
 __global__ void k_set_plan(SortPlan* plan, SortSeg* seg, int passes, const int32_t* digits,
-                           uint32_t m) {
+                           uint32_t m, int items) {
   seg->key_base = 0;  // one segment: all the keys
   seg->count = m;
   seg->block_base = 0;
-  seg->nblocks = (m + kSortTile - 1) / kSortTile;
+  seg->nblocks = (m + 256u * items - 1) / (256u * items);
   plan->blocks = seg->nblocks;
+  plan->items = items;
   plan->passes = passes;
   for (int k = 0; k < 8; ++k) {
     plan->shift[k] = k < passes ? 8 * digits[k] : 0;
@@ -933,14 +939,15 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
 // after a host read-back, reserved in a graph, or the buffers' present size
 // ahead of the read-back; k_plan and k_emit flag a frame that exceeds them).
 static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
-                                uint64_t v_cap, uint64_t m_cap, hipStream_t st) {
+                                uint64_t v_cap, uint64_t m_cap, int force_items,
+                                hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
     const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 256 * 16);
     hipLaunchKernelGGL(k_plan, dim3(pb), dim3(256), 0, st, c->packed.as<uint64_t>(),
                        c->totals.as<uint64_t>(), frame_bounds(c), sd.n, (int)fs.nviews, m_cap,
                        v_cap, plan, c->spans.as<ViewSpan>(), c->segs.as<SortSeg>(),
-                       c->ranges.as<int2>(), (uint32_t)fs.tiles);
+                       c->ranges.as<int2>(), (uint32_t)fs.tiles, force_items);
     LUMEN_LAUNCHED();
   } else {
     LUMEN_TRY(fill<uint32_t>(c->spans.as<uint32_t>(), 0u, 4ull * fs.nviews, st));
@@ -963,8 +970,8 @@ static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const Fra
 // Stage 3 of a frame: sort, ranges and compositing of at most m_cap keys.
 static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
                                   const LumenExRenderConfig* cfg, uint64_t m_cap, int sort_slots,
-                                  int sort_modes, bool keep, float* out_rgb, float* out_depth,
-                                  hipStream_t st) {
+                                  int sort_modes, int items, bool keep, float* out_rgb,
+                                  float* out_depth, hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0 && m_cap > 0) {
     if (keep) {
@@ -979,7 +986,7 @@ static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const F
     uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
     LUMEN_TRY(radix_sort_planned(kb, vb, plan, c->segs.as<SortSeg>(), (int)fs.nviews, m_cap,
-                                 sort_slots, sort_modes,
+                                 sort_slots, sort_modes, items,
                                  SortScratch{&c->hist, &c->hist_part}, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
@@ -1008,9 +1015,10 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                                 const LumenExRenderConfig* cfg, uint64_t v_cap, uint64_t m_cap,
                                 int sort_slots, int sort_modes, bool keep, float* out_rgb,
                                 float* out_depth, hipStream_t st) {
-  LUMEN_TRY(enqueue_emit(c, sd, fs, v_cap, m_cap, st));
-  return enqueue_finish(c, sd, fs, cfg, m_cap, sort_slots, sort_modes, keep, out_rgb, out_depth,
-                        st);
+  const int items = sort_items_for(m_cap);
+  LUMEN_TRY(enqueue_emit(c, sd, fs, v_cap, m_cap, items, st));
+  return enqueue_finish(c, sd, fs, cfg, m_cap, sort_slots, sort_modes, items, keep, out_rgb,
+                        out_depth, st);
 }
 
 // The visible-splat and key capacities the buffers hold now (0 before a
@@ -1073,7 +1081,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
     // flagged by k_plan / k_emit and both run again below.
     current_capacity(c, v_now, m_now);
     if (v_now > 0 && m_now > 0) {
-      LUMEN_TRY(enqueue_emit(c, sd, fs, v_now, m_now, st));
+      LUMEN_TRY(enqueue_emit(c, sd, fs, v_now, m_now, 0, st));
       ahead = true;
     }
     LUMEN_HIP(hipEventSynchronize(c->ev_totals));
@@ -1092,7 +1100,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
     // the buffers may be replaced: the emission enqueued above must be done
     if (ahead) LUMEN_HIP(hipStreamSynchronize(st));
     LUMEN_TRY(ensure_capacity(c, V, M));
-    LUMEN_TRY(enqueue_emit(c, sd, fs, V, M, st));
+    LUMEN_TRY(enqueue_emit(c, sd, fs, V, M, 0, st));
   }
   // the plan k_plan will write, known here from the bounds: only its passes
   // and key width are enqueued
@@ -1100,8 +1108,8 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   plan_digits(V > 0, hb[0], hb[1], hb[2], hb[3], hp);
   const int np = hp.passes;
   const bool keep = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0 && M > 0;
-  LUMEN_TRY(enqueue_finish(c, sd, fs, cfg, M, np, hp.mode == 1 ? 2 : 1, keep, out_rgb,
-                           out_depth, st));
+  LUMEN_TRY(enqueue_finish(c, sd, fs, cfg, M, np, hp.mode == 1 ? 2 : 1, sort_items_for(M), keep,
+                           out_rgb, out_depth, st));
   c->V = V;
   c->M = M;
   c->T = fs.tiles;
@@ -1176,8 +1184,9 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
   // sort scratch for the capacity, allocated before capture
-  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap, (int)nviews) * sizeof(uint32_t)));
-  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_cap, (int)nviews)));
+  const int items = sort_items_for(m_cap);
+  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap, (int)nviews, items) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_cap, (int)nviews, items)));
   LUMEN_HIP(hipStreamSynchronize(st));
   const int slots = max_sort_slots(c, fs);
   const SceneDev sd = scene_dev(sc);
@@ -1269,8 +1278,9 @@ static LumenStatus workspace(LumenExContext* c, Need&& need, uint64_t N, const W
   LUMEN_TRY(frame_buffers(c, need, N, ws.fs, save));
   LUMEN_TRY(capacity_buffers(c, need, v_cap, m_cap));
   if (m_cap > 0) {
-    LUMEN_TRY(need(c->hist, sort_hist_len(m_cap, (int)ws.fs.nviews) * sizeof(uint32_t)));
-    LUMEN_TRY(need(c->hist_part, sort_part_bytes(m_cap, (int)ws.fs.nviews)));
+    // the larger table of the two tile sizes a frame of up to m_cap keys may use
+    LUMEN_TRY(need(c->hist, sort_hist_len(m_cap, (int)ws.fs.nviews, 12) * sizeof(uint32_t)));
+    LUMEN_TRY(need(c->hist_part, sort_part_bytes(m_cap, (int)ws.fs.nviews, 12)));
   }
   if ((flags & LUMEN_EX_KEEP_STAGES) && m_cap > 0) {
     LUMEN_TRY(need(c->keys_keep, m_cap * 8));
@@ -1508,13 +1518,14 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   memcpy(static_cast<uint8_t*>(c->pinned) + 64, digits, sizeof(digits));
   LUMEN_HIP(hipMemcpyAsync(ddig, static_cast<uint8_t*>(c->pinned) + 64, sizeof(digits),
                            hipMemcpyHostToDevice, st));
+  const int items = sort_items_for(n);
   hipLaunchKernelGGL(k_set_plan, dim3(1), dim3(1), 0, st, c->plan.as<SortPlan>(),
-                     c->segs.as<SortSeg>(), nd, ddig, (uint32_t)n);
+                     c->segs.as<SortSeg>(), nd, ddig, (uint32_t)n, items);
   LUMEN_LAUNCHED();
   uint64_t* kb[2] = {keys, c->keys[1].as<uint64_t>()};
   uint32_t* vb[2] = {values, c->vals[1].as<uint32_t>()};
   LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), c->segs.as<SortSeg>(), 1, n, nd,
-                               1, SortScratch{&c->hist, &c->hist_part}, st));
+                               1, items, SortScratch{&c->hist, &c->hist_part}, st));
   if (nd & 1) {
     LUMEN_HIP(hipMemcpyAsync(keys, kb[1], n * 8, hipMemcpyDeviceToDevice, st));
     LUMEN_HIP(hipMemcpyAsync(values, vb[1], n * 4, hipMemcpyDeviceToDevice, st));

@@ -14,8 +14,9 @@
 //                   digit-major (SortSeg) so that one exclusive scan gives
 //                   every block's start offset per digit
 //   exclusive_scan  over the kDigitBins x blocks table
-//   k_sort_scatter  stable rank inside the block: each wave owns 1024
-//                   consecutive keys and ranks them 64 at a time with one
+//   k_sort_scatter  stable rank inside the block (256 threads of 12 or 16
+//                   keys, sort_items_for): each wave owns a quarter of the
+//                   block's consecutive keys and ranks them 64 at a time with one
 //                   ballot per digit bit (lanes with equal digits are peers; rank = peers
 //                   below this lane) and wave-private digit counters in LDS;
 //                   one block scan then orders the waves, the block's keys are
@@ -31,8 +32,6 @@
 namespace lumen {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
 
 // The blocks of a launch are dealt to the 8 XCDs in turn (block i runs on
 // XCD i % 8), each XCD with its own L2.  xcd_block renumbers blocks 0..n-1 so
@@ -81,14 +80,15 @@ __device__ __forceinline__ bool sort_mode_is(const SortPlan* plan) {
   return plan->mode == (sizeof(K) == 4 ? 1 : 0);
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K* k1,
                                                            const SortPlan* __restrict__ plan,
                                                            const SortSeg* __restrict__ segs,
                                                            int nseg, int pass,
                                                            uint32_t* __restrict__ hist) {
+  constexpr int kTile = kSortThreads * ITEMS;
   __shared__ uint32_t h[4][kDigitBins];
-  if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
+  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
   SortSeg sg;
   uint32_t lb;
   if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
@@ -99,18 +99,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
   const int w = t >> 6;
   for (int d = t; d < 4 * kDigitBins; d += kSortThreads) (&h[0][0])[d] = 0u;
   __syncthreads();
-  const uint32_t base = sg.key_base + lb * kSortTile;
+  const uint32_t base = sg.key_base + lb * kTile;
   const uint32_t end = sg.key_base + sg.count;
   // all loads first (a load feeding an atomic in the same step would wait
   // for each key in turn)
-  K kk[kSortItems];
+  K kk[ITEMS];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const uint32_t idx = base + j * kSortThreads + t;
     kk[j] = idx < end ? keys[idx] : K(0);
   }
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     if (base + j * kSortThreads + t < end)
       atomicAdd(&h[w][(uint32_t)(kk[j] >> shift) & mask], 1u);
   }
@@ -119,20 +119,21 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
     hist[hist_index(sg, (uint32_t)d, lb)] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     K* k0, K* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
     const SortSeg* __restrict__ segs, int nseg, int pass, const uint32_t* __restrict__ offsets) {
-  __shared__ K s_key[kSortTile];
-  __shared__ uint32_t s_val[kSortTile];
-  // per-wave digit counts, then per-wave starts (both <= kSortTile; 16 bits
+  constexpr int kTile = kSortThreads * ITEMS;
+  __shared__ K s_key[kTile];
+  __shared__ uint32_t s_val[kTile];
+  // per-wave digit counts, then per-wave starts (both <= kTile; 16 bits
   // keep the block's LDS under 40 KB with 32-bit keys: four blocks per CU)
   __shared__ uint16_t s_cnt[4][kDigitBins];
   // output position of the block's key at LDS slot i of digit d: s_delta[d] + i
   // (the digit's global start minus its block-local start, modulo 2^32)
   __shared__ uint32_t s_delta[kDigitBins];
   __shared__ uint32_t s_w[4];
-  if (pass >= plan->passes || !sort_mode_is<K>(plan)) return;
+  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
   SortSeg sg;
   uint32_t lb;
   if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
@@ -153,14 +154,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     for (int i = 0; i < 4; ++i) s_cnt[i][d] = 0;
   }
   __syncthreads();
-  // wave w owns keys [base + 1024 w, base + 1024 (w + 1)), 64 per round
-  const uint32_t base = sg.key_base + lb * kSortTile;
-  const uint32_t wbase = base + (uint32_t)w * (kSortTile / 4);
-  K kk[kSortItems];
-  uint32_t vv[kSortItems];
-  uint32_t off[kSortItems];
+  // wave w owns keys [base + w kTile/4, base + (w + 1) kTile/4), 64 per round
+  const uint32_t base = sg.key_base + lb * kTile;
+  const uint32_t wbase = base + (uint32_t)w * (kTile / 4);
+  K kk[ITEMS];
+  uint32_t vv[ITEMS];
+  uint32_t off[ITEMS];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const uint32_t idx = wbase + (uint32_t)j * 64u + (uint32_t)lane;
     kk[j] = idx < m ? kin[idx] : K(0);
     vv[j] = idx < m ? vin[idx] : 0u;
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
 #pragma unroll
   for (int e = 0; e < kPer; ++e) glob[e] = offsets[hist_index(sg, (uint32_t)(kPer * t + e), lb)];
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
     const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
     uint64_t peers = __ballot(valid);
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
+  for (int j = 0; j < ITEMS; ++j) {
     if (wbase + (uint32_t)j * 64u + (uint32_t)lane < m) {
       const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
       const uint32_t pos = s_cnt[w][d] + off[j];
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   __syncthreads();
   // the block's keys in digit order: each digit's run goes to consecutive
   // addresses of the output
-  const uint32_t nblk = min((uint32_t)kSortTile, m - base);
+  const uint32_t nblk = min((uint32_t)kTile, m - base);
   for (uint32_t i = t; i < nblk; i += kSortThreads) {
     const K key = s_key[i];
     const uint32_t d = (uint32_t)(key >> shift) & mask;
@@ -245,42 +246,33 @@ struct SortScratch {
 
 // Sort blocks (the grid) for up to m_cap keys in nseg segments, the
 // histogram table (one row per digit per block) and the scan's partial sums.
-inline uint32_t sort_grid(uint64_t m_cap, int nseg) {
-  return div_up(m_cap, kSortTile) + (uint32_t)nseg;
+inline uint32_t sort_grid(uint64_t m_cap, int nseg, int items) {
+  return div_up(m_cap, (uint64_t)kSortThreads * items) + (uint32_t)nseg;
 }
-inline uint64_t sort_hist_len(uint64_t m_cap, int nseg) {
-  return (uint64_t)sort_grid(m_cap, nseg) * kDigitBins;
+inline uint64_t sort_hist_len(uint64_t m_cap, int nseg, int items) {
+  return (uint64_t)sort_grid(m_cap, nseg, items) * kDigitBins;
 }
-inline size_t sort_part_bytes(uint64_t m_cap, int nseg) {
-  return (size_t)div_up(sort_hist_len(m_cap, nseg), kScanChunk) * sizeof(uint32_t);
+inline size_t sort_part_bytes(uint64_t m_cap, int nseg, int items) {
+  return (size_t)div_up(sort_hist_len(m_cap, nseg, items), kScanChunk) * sizeof(uint32_t);
 }
 
-// Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
-// memory; the result ends in buffer (plan->passes & 1), as 64-bit keys
-// (mode 0) or 32-bit frame keys (mode 1) in the same buffers.  `modes` says
-// which key widths to enqueue (bit 0: 64-bit, bit 1: 32-bit; a graph, which
-// cannot know the frame's mode, takes both).  m_cap bounds the number of
-// keys the plan may name.
-inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
-                                      const SortSeg* segs, int nseg, uint64_t m_cap, int slots,
-                                      int modes, SortScratch sc, hipStream_t st) {
-  if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
-  const uint32_t nb = sort_grid(m_cap, nseg);
-  const uint64_t hlen = sort_hist_len(m_cap, nseg);
-  LUMEN_TRY(ensure(*sc.hist, hlen * sizeof(uint32_t)));
-  LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap, nseg)));
-  uint32_t* hist = sc.hist->as<uint32_t>();
-  uint32_t* part = sc.hist_part->as<uint32_t>();
+// The passes of one tile size: `slots` pass slots over (k[0], v[0]).
+template <int ITEMS>
+inline LumenStatus sort_passes(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
+                               const SortSeg* segs, int nseg, uint64_t m_cap, int slots, int modes,
+                               uint32_t* hist, uint32_t* part, hipStream_t st) {
+  const uint32_t nb = sort_grid(m_cap, nseg, ITEMS);
+  const uint64_t hlen = sort_hist_len(m_cap, nseg, ITEMS);
   uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};
   for (int p = 0; p < slots; ++p) {
     if (modes & 1) {
-      hipLaunchKernelGGL(k_sort_hist<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0], k[1],
-                         plan, segs, nseg, p, hist);
+      hipLaunchKernelGGL((k_sort_hist<uint64_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st, k[0],
+                         k[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
     if (modes & 2) {
-      hipLaunchKernelGGL(k_sort_hist<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
-                         k32[1], plan, segs, nseg, p, hist);
+      hipLaunchKernelGGL((k_sort_hist<uint32_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st,
+                         k32[0], k32[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
     // the pass's scan runs only when the device plan has the pass, and over
@@ -288,17 +280,38 @@ inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const Sort
     LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st,
                                        ScanGate{&plan->passes, p, &plan->blocks, kDigitBins}));
     if (modes & 1) {
-      hipLaunchKernelGGL(k_sort_scatter<uint64_t>, dim3(nb), dim3(kSortThreads), 0, st, k[0],
-                         k[1], v[0], v[1], plan, segs, nseg, p, hist);
+      hipLaunchKernelGGL((k_sort_scatter<uint64_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st,
+                         k[0], k[1], v[0], v[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
     if (modes & 2) {
-      hipLaunchKernelGGL(k_sort_scatter<uint32_t>, dim3(nb), dim3(kSortThreads), 0, st, k32[0],
-                         k32[1], v[0], v[1], plan, segs, nseg, p, hist);
+      hipLaunchKernelGGL((k_sort_scatter<uint32_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st,
+                         k32[0], k32[1], v[0], v[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
   }
   return LUMEN_SUCCESS;
+}
+
+// Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
+// memory; the result ends in buffer (plan->passes & 1), as 64-bit keys
+// (mode 0) or 32-bit frame keys (mode 1) in the same buffers.  `modes` says
+// which key widths to enqueue (bit 0: 64-bit, bit 1: 32-bit; a graph, which
+// cannot know the frame's mode, takes both).  m_cap bounds the number of
+// keys the plan may name; `items` (12 or 16) is the plan's keys per thread,
+// which the segments' block counts were made with.
+inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
+                                      const SortSeg* segs, int nseg, uint64_t m_cap, int slots,
+                                      int modes, int items, SortScratch sc, hipStream_t st) {
+  if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
+  if (items != 12 && items != 16) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "sort: %d keys per thread", items);
+  LUMEN_TRY(ensure(*sc.hist, sort_hist_len(m_cap, nseg, items) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap, nseg, items)));
+  uint32_t* hist = sc.hist->as<uint32_t>();
+  uint32_t* part = sc.hist_part->as<uint32_t>();
+  if (items == 12)
+    return sort_passes<12>(k, v, plan, segs, nseg, m_cap, slots, modes, hist, part, st);
+  return sort_passes<16>(k, v, plan, segs, nseg, m_cap, slots, modes, hist, part, st);
 }
 
 }  // namespace lumen

@@ -231,7 +231,10 @@ def test_radix_sort_composite_kat(rasterizer, oracle):
     assert v.tolist() == [2, 1, 3, 0]
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 3_000_001])
+# 2^25 + 1 keys: past the size where a sort block takes 16 keys per thread
+# instead of 12 (block edges 3072 apart below it, 4096 above)
+@pytest.mark.parametrize("n", [0, 1, 2, 3071, 3072, 3073, 4095, 4096, 4097, 3_000_001,
+                               (1 << 25) + 1])
 def test_radix_sort_matches_stable_argsort(rasterizer, n):
     rng = np.random.default_rng(n)
     hi = rng.integers(0, 1 << 13, n).astype(np.uint64) << np.uint64(32)
