@@ -165,8 +165,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       s0[j] = a;                                // x, y, i00, 2 i01
       s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
       s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, tx * v.tile_w, ty * v.tile_h,
-                                      v.tile_w, rows, nwaves);
+      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                      tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
     }
     __syncthreads();
     // this wave's list: splats up to its pixels' last evaluated one whose box

@@ -618,8 +618,8 @@ __global__ __launch_bounds__(256) void k_composite(
         s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
         s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
         s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, tx * v.tile_w, ty * v.tile_h,
-                                        v.tile_w, rows, nwaves);
+        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                        tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
       }
       __syncthreads();
       // this wave's list of the batch's splats, in order

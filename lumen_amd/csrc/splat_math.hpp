@@ -352,21 +352,65 @@ __device__ __forceinline__ void limit_box(float lim, float& hx, float& hy) {
   hy = (hy - 0.02f) * s + 0.02f;
 }
 
-// Bit w of the result: the splat's box (centre +- (hx, hy), outside of which
-// its power exceeds 4.5 at every pixel centre) meets the pixel centres of
-// wave w's strip of the tile (64 / tile_w full rows).  A wave whose bit is
-// clear computes power > 4.5 (alpha 0, no blend) for the splat at each of
-// its pixels, so skipping the splat there changes nothing.
-__device__ __forceinline__ uint32_t strip_mask(float sx, float sy, float hx, float hy, int tile_x0,
+// The smallest power 0.5 (i00 dx^2 + 2 i01 dx dy + i11 dy^2), d = p - (sx,
+// sy), over the rectangle [x0, x1] x [y0, y1]: 0 with the centre inside,
+// else the least of the four edges' minima (the power is convex, so on an
+// edge it is least at its stationary point clamped to the edge).
+__device__ __forceinline__ float rect_min_power(float sx, float sy, float i00, float i01, float i11,
+                                                float x0, float x1, float y0, float y1) {
+  if (sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1) return 0.0f;
+  auto q = [&](float dx, float dy) {
+    return 0.5f * (i00 * dx * dx + 2.0f * i01 * dx * dy + i11 * dy * dy);
+  };
+  const float r00 = 1.0f / i00, r11 = 1.0f / i11;
+  float m = __builtin_inff();
+  const float ys[2] = {y0, y1}, xs[2] = {x0, x1};
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dy = ys[e] - sy;  // edge y = ys[e]: least at dx = -i01 dy / i00
+    const float dx = fminf(fmaxf(-i01 * dy * r00, x0 - sx), x1 - sx);
+    m = fminf(m, q(dx, dy));
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dx = xs[e] - sx;  // edge x = xs[e]: least at dy = -i01 dx / i11
+    const float dy = fminf(fmaxf(-i01 * dx * r11, y0 - sy), y1 - sy);
+    m = fminf(m, q(dx, dy));
+  }
+  return m;
+}
+
+// Bit w of the result: splat (centre (sx, sy), inverse covariance i00, i01,
+// i11, power limit lim) can reach lim at a pixel centre of wave w's strip of
+// the tile (64 / tile_w full rows).  A wave whose bit is clear computes
+// power > lim (no blend) for the splat at each of its pixels, so skipping
+// the splat there changes nothing.  First the splat's box (centre +- (hx,
+// hy), outside of which its power exceeds the limit at every pixel centre);
+// then, for conics with condition number up to 1e4 (the box's own bound),
+// the power's minimum over the strip's rectangle of pixel centres, which
+// must exceed 1.02 lim + 1e-3 to clear the bit: a margin far above the
+// rounding of either evaluation.  A rotated, elongated splat's box meets
+// strips its ellipse misses (14% of the box's (splat, strip) pairs at
+// 1M/1080p).
+__device__ __forceinline__ uint32_t strip_mask(float sx, float sy, float hx, float hy, float i00,
+                                               float i01, float i11, float lim, int tile_x0,
                                                int tile_y0, int tile_w, int rows, int nwaves) {
   const float x0 = (float)tile_x0 + 0.5f;
   const float x1 = (float)(tile_x0 + tile_w) - 0.5f;
   if (sx + hx < x0 || sx - hx > x1) return 0u;
+  const float det = i00 * i11 - i01 * i01;
+  const float tr = i00 + i11;
+  // (a NaN centre or limit keeps the box's answer: such a splat blends NaN)
+  const bool exact = i00 > 0.0f && i11 > 0.0f && det > 0.0f && tr * tr <= 1.0e4f * det &&
+                     sx == sx && sy == sy && lim == lim;
+  const float cut = 1.02f * lim + 1e-3f;
   uint32_t m = 0u;
   for (int w = 0; w < nwaves; ++w) {
     const float y0 = (float)(tile_y0 + w * rows) + 0.5f;
     const float y1 = (float)(tile_y0 + w * rows + rows) - 0.5f;
-    if (!(sy + hy < y0 || sy - hy > y1)) m |= 1u << w;
+    if (sy + hy < y0 || sy - hy > y1) continue;
+    if (exact && rect_min_power(sx, sy, i00, i01, i11, x0, x1, y0, y1) > cut) continue;
+    m |= 1u << w;
   }
   return m;
 }
