@@ -167,6 +167,31 @@ def test_camera_inside_cloud_and_large_splats(rasterizer, oracle):
     compare(rasterizer, oracle, sc, cam)
 
 
+def test_thin_rotated_splats(rasterizer, oracle):
+    """Needle-like splats (axis ratios 20:1 to 1000:1) at every angle, with
+    opacities down to the 1/255 limit: their boxes meet strips their
+    ellipses miss, and their conics straddle the condition-number bound up
+    to which the compositors test the ellipse exactly.  Every stage and
+    pixel stays the oracle's, in both tile shapes with two and one pixel per
+    lane."""
+    rng = np.random.default_rng(31)
+    n = 4000
+    pos = rng.uniform(-1.5, 1.5, (n, 3))
+    pos[:, 2] *= 0.3
+    long_ = np.exp(rng.uniform(math.log(0.05), math.log(1.2), n))
+    ratio = np.exp(rng.uniform(math.log(20.0), math.log(1000.0), n))
+    scale = np.stack([long_, long_ / ratio, long_ / ratio], axis=1)
+    ang = rng.uniform(0, math.pi, n)  # rotation about the view axis (z)
+    rot = np.stack([np.cos(ang / 2), np.zeros(n), np.zeros(n), np.sin(ang / 2)], axis=1)
+    rot[n // 2:] = rng.standard_normal((n - n // 2, 4))  # and arbitrary 3D ones
+    op = np.exp(rng.uniform(math.log(1.0 / 255.0), 0.0, n)).clip(0.0, 0.99)
+    sh = rng.normal(0, 0.3, (n, 48))
+    sc = GaussianScene.from_arrays(pos, scale, rot, op, sh)
+    cam = look_at(CameraConfig(640, 480, 500.0, 500.0, 320.0, 240.0, 0.01, 100.0), (0.0, 0.0, 4.0))
+    for tw, th in ((16, 16), (8, 8)):
+        compare(rasterizer, oracle, sc, cam, RenderConfig(tile_width=tw, tile_height=th))
+
+
 def test_unnormalized_and_degenerate_inputs(rasterizer, oracle):
     """Raw quaternions of any norm (incl. ~0 -> identity), scales below the
     1e-5 clamp, opacity at the 1/255 cull threshold."""
