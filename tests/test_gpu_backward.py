@@ -76,6 +76,42 @@ def test_backward_matches_fp64_oracle(oracle, white, depth_on):
     assert rel_err(vs, ref["viewspace"]) < 2e-3
 
 
+def test_backward_thin_rotated_splats(oracle):
+    """Needle-like splats at every angle (axis ratios 20:1 to 1000:1, the
+    2D conics' conditioning around the bound up to which the compositors test
+    each wave's strip against the ellipse exactly): the backward's 4-row
+    strips drop no splat that reaches a pixel, so the gradients stay the
+    FP64 oracle's."""
+    rng = np.random.default_rng(41)
+    n = 1500
+    pos = rng.uniform(-1.0, 1.0, (n, 3))
+    pos[:, 2] *= 0.3
+    long_ = np.exp(rng.uniform(math.log(0.05), math.log(1.2), n))
+    ratio = np.exp(rng.uniform(math.log(20.0), math.log(1000.0), n))
+    scale = np.stack([long_, long_ / ratio, long_ / ratio], axis=1)
+    ang = rng.uniform(0, math.pi, n)
+    rot = np.stack([np.cos(ang / 2), np.zeros(n), np.zeros(n), np.sin(ang / 2)], axis=1)
+    rot[n // 2:] = rng.standard_normal((n - n // 2, 4))
+    op = np.exp(rng.uniform(math.log(1.0 / 255.0), 0.0, n)).clip(0.0, 0.99)
+    sh = rng.normal(0, 0.08, (n, 48))
+    sh[:, :3] = rng.normal(0.2, 0.4, (n, 3))
+    scene = GaussianScene.from_arrays(pos, scale, rot, op, sh)
+    cam = look_at(CameraConfig(160, 120, 150.0, 150.0, 80.0, 60.0, 0.01, 1000.0), (0.0, 0.0, 3.0))
+    cfg = RenderConfig()
+    g = np.random.default_rng(2)
+    g_rgb = g.normal(0, 1, (120, 160, 3))
+    g_d = g.normal(0, 0.2, (120, 160))
+    say("[bwd] thin rotated splats")
+    gp, gs, vs, cnt = gpu_grads(scene, cam, cfg, g_rgb, g_d)
+    ref = oracle.backward64(scene.planes, scene.sh, cam, cfg, g_rgb, g_d)
+    for name, sl in GROUPS.items():
+        e = rel_err(gp[sl], ref["planes"][sl])
+        say(f"[bwd]   {name}: rel L2 err {e:.2e}")
+        assert e < 2e-3, name
+    assert rel_err(gs, ref["sh"]) < 2e-3
+    np.testing.assert_array_equal(cnt, ref["count"])
+
+
 def test_backward_against_fp64_finite_differences(oracle):
     """GPU gradient vs central differences of the FP64 forward, on a sample of
     parameters of a 2k-splat scene; samples whose renders change a discrete
