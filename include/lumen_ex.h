@@ -174,7 +174,7 @@ LumenStatus lumen_ex_frame_status(LumenExContext* ctx, LumenExViewStats* out_sta
 LumenStatus lumen_ex_radix_sort(LumenExContext* ctx, uint64_t* keys, uint32_t* values,
                                 uint64_t n, void* stream);
 
-/* TileRangeIndex.build_from_sorted_keys (src/raster/binning.mojo:306-328):
+/* TileRangeIndex.build_from_sorted_keys (src/raster/binning.mojo:51-73):
  * ranges[2*t], ranges[2*t+1] = [start, end) of tile t in the sorted keys, or
  * (-1, -1) when the tile has no instance. */
 LumenStatus lumen_ex_build_ranges(const uint64_t* sorted_keys, uint64_t n, uint32_t tiles,

@@ -71,24 +71,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link of liblumen.so failed")
     os.replace(tmp, LIB)
-    build_probe(verbose)
     return LIB
-
-
-PROBE = os.path.join(ROOT, "lumen_amd", "probe_valu")
-
-
-def build_probe(verbose: bool = False) -> str:
-    """The VALU issue-rate probe (lumen_amd/csrc/probe_valu.hip), a standalone
-    program the roofline notes read the compositors against."""
-    cmd = [hipcc(), *HIPCC_FLAGS, os.path.join(CSRC, "probe_valu.hip"), "-o", PROBE]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed on probe_valu.hip")
-    return PROBE
 
 
 def build_oracle() -> None:

@@ -251,7 +251,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
 }
 
 // Radius exactly as splat_geometry computes it (solve_eigenvalues_2x2,
-// src/math/cholesky.mojo:287-310), from the cached screen covariance.
+// src/math/cholesky.mojo:28-51), from the cached screen covariance.
 __device__ __forceinline__ int32_t radius_of(float c00, float c01, float c11) {
   const float mid = (c00 + c11) * 0.5f;
   const float diff = (c00 - c11) * 0.5f;

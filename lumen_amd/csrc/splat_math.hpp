@@ -134,20 +134,20 @@ struct SplatGeom {
 // Returns true when the Gaussian survives every cull and touches a tile.
 LUMEN_HD bool splat_geometry(const ViewParams& v, float px, float py, float pz, float s0, float s1,
                              float s2, Quat graw, float opacity, SplatGeom& g) {
-  // FrustumCuller.is_opacity_visible, src/raster/culling.mojo:144-148
+  // FrustumCuller.is_opacity_visible, src/raster/culling.mojo:47-51
   if (!(opacity >= kMinAlpha)) return false;
   // world_to_cam, src/scene/camera.mojo:69-71 + quaternion.mojo:80-87
   Quat p{0.0f, px - v.eye[0], py - v.eye[1], pz - v.eye[2]};
   Quat r = quat_mul(quat_mul(v.q_view, p), v.q_view_inv);
   const float tx = r.x, ty = r.y, tz = r.z;
-  // is_depth_visible, culling.mojo:138-142
+  // is_depth_visible, culling.mojo:41-45
   if (!(tz >= v.near_plane && tz <= v.far_plane)) return false;
   // cam_to_screen, camera.mojo:73-79
   float z = tz;
   if (z < 0.0001f && z > -0.0001f) z = 0.0001f;
   const float sx = v.fx * (tx / z) + v.cx;
   const float sy = v.fy * (ty / z) + v.cy;
-  // Covariance3D.from_scale_and_rotation, src/math/covariance.mojo:189-231;
+  // Covariance3D.from_scale_and_rotation, src/math/covariance.mojo:42-83;
   // the Gaussian's rotation is normalized at fetch (gaussian.mojo:36) and
   // again inside to_rotation_matrix.
   float R[9];
@@ -187,14 +187,14 @@ LUMEN_HD bool splat_geometry(const ViewParams& v, float px, float py, float pz, 
   const float c00 = q00 * t00 + q01 * t01 + q02 * t02 + v.lowpass;
   const float c01 = q00 * t10 + q01 * t11 + q02 * t12;
   const float c11 = q10 * t10 + q11 * t11 + q12 * t12 + v.lowpass;
-  // solve_eigenvalues_2x2, src/math/cholesky.mojo:287-310
+  // solve_eigenvalues_2x2, src/math/cholesky.mojo:28-51
   const float mid = (c00 + c11) * 0.5f;
   const float diff = (c00 - c11) * 0.5f;
   const float root = sqrtf(diff * diff + c01 * c01);
   const float l_max = mid + root;
   const int64_t rad = trunc_to_i64(ceilf(3.0f * sqrtf(l_max)));
   if (rad < 1) return false;  // pipeline.mojo:108
-  // is_screen_box_visible, culling.mojo:150-168
+  // is_screen_box_visible, culling.mojo:53-71
   const float rf = (float)rad;
   if (sx + rf < 0.0f) return false;
   if (sx - rf >= (float)v.width) return false;

@@ -189,7 +189,7 @@ void orc_cam_to_screen(const orc_camera *cam, const float pc[3], float out[2]) {
 /* ------------------------------------------------------------------------ */
 /* Covariance / EWA / eigen: src/math/{covariance,projection,cholesky}.mojo  */
 /* ------------------------------------------------------------------------ */
-void orc_cov3d(const float scale[3], const float qin[4], float c[6]) { /* covariance.mojo:189-231 */
+void orc_cov3d(const float scale[3], const float qin[4], float c[6]) { /* covariance.mojo:42-83 */
     float sx = scale[0]; if (sx < 0.00001f) sx = 0.00001f;
     float sy = scale[1]; if (sy < 0.00001f) sy = 0.00001f;
     float sz = scale[2]; if (sz < 0.00001f) sz = 0.00001f;
@@ -242,7 +242,7 @@ void orc_ewa(const float s[6], const float pc[3], const float W[9], float fx,
     out[2] = q10 * t10 + q11 * t11 + q12 * t12 + lowpass;
 }
 
-int64_t orc_eigen(const float cov[3], float lam[2]) {   /* cholesky.mojo:287-310 */
+int64_t orc_eigen(const float cov[3], float lam[2]) {   /* cholesky.mojo:28-51 */
     float a = cov[0], b = cov[1], c = cov[2];
     float mid = (a + c) * 0.5f;
     float diff = (a - c) * 0.5f;
@@ -318,15 +318,15 @@ void orc_sh_color(const float *sh, const float dir[3], int64_t degree, float out
 /* ------------------------------------------------------------------------ */
 /* Culling and tiles: src/raster/{culling,tile}.mojo                         */
 /* ------------------------------------------------------------------------ */
-int orc_depth_visible(float near_plane, float far_plane, float z) { /* culling.mojo:138-142 */
+int orc_depth_visible(float near_plane, float far_plane, float z) { /* culling.mojo:41-45 */
     return z >= near_plane && z <= far_plane;
 }
 
-int orc_opacity_visible(float op) {                      /* culling.mojo:144-148 */
+int orc_opacity_visible(float op) {                      /* culling.mojo:47-51 */
     return op >= MIN_ALPHA_THRESHOLD;
 }
 
-int orc_screen_box_visible(int64_t w, int64_t h, float cx, float cy, int64_t r) { /* :150-168 */
+int orc_screen_box_visible(int64_t w, int64_t h, float cx, float cy, int64_t r) { /* culling.mojo:53-71 */
     float rf = (float)r;
     if (cx + rf < 0.0f) return 0;
     if (cx - rf >= (float)w) return 0;
@@ -391,7 +391,7 @@ void orc_radix_sort(uint64_t *keys, int64_t *vals, int64_t n) { /* radix_sort.mo
     free(vt);
 }
 
-void orc_build_ranges(const uint64_t *keys, int64_t n, int64_t tiles, int64_t *ranges) { /* binning.mojo:306-328 */
+void orc_build_ranges(const uint64_t *keys, int64_t n, int64_t tiles, int64_t *ranges) { /* binning.mojo:51-73 */
     for (int64_t t = 0; t < tiles; ++t) { ranges[2 * t] = -1; ranges[2 * t + 1] = -1; }
     if (n == 0) return;
     int64_t prev = -1;

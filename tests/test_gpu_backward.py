@@ -175,7 +175,7 @@ def test_losses_match_reference_semantics(oracle):
     g = np.zeros(a.size, np.float32)
     ref_l1 = oracle.lib().orc_l1_loss(oracle.fptr(a.ravel().copy()), oracle.fptr(b.ravel().copy()),
                                       a.size, oracle.fptr(g))
-    assert float(l1) == pytest.approx(ref_l1, rel=1e-5)
+    assert float(l1.detach()) == pytest.approx(ref_l1, rel=1e-5)
     np.testing.assert_allclose(ta.grad.cpu().numpy().ravel(), g, rtol=1e-6)
     s = float(ssim_window(ta.detach(), tb))
     ref_s = oracle.lib().orc_ssim(oracle.fptr(a.ravel().copy()), oracle.fptr(b.ravel().copy()), 56, 40, 8)

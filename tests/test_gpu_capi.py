@@ -154,7 +154,7 @@ def test_framebuffer_matches_oracle(oracle, eye):
     a_ref = 1.0 - ref["t_final"]
     assert np.max(np.abs(alpha - np.where(a_ref > 0, a_ref, 0.0))) <= 1e-4
     hit = a_ref > 1e-3
-    np.testing.assert_allclose(depth[hit], (ref["depth"] / a_ref)[hit], rtol=1e-3)
+    np.testing.assert_allclose(depth[hit], ref["depth"][hit] / a_ref[hit], rtol=1e-3)
     assert np.all(depth[a_ref <= 0] == 1000.0)
     # a second frame composites behind the first
     a0 = alpha.copy()
