@@ -474,7 +474,16 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const float* __restrict__ grad_vis,
                                                         GradPlanes out, bool assign) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= s.n || plan->overflow) return;
+  if (i >= s.n) return;
+  if (plan->overflow) {  // the frame was not rendered: no gradient
+    if (assign) {
+      for (int k = 0; k < 11; ++k) out.p[k][i] = 0.0f;
+      float4* z4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
+#pragma unroll
+      for (int qd = 0; qd < 12; ++qd) z4[qd] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return;
+  }
   // the gradient planes are added to, or written (assign)
   auto put = [&](int plane, float g) {
     if (assign) out.p[plane][i] = g;
@@ -734,7 +743,16 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   // Sizes: exact after a plain frame, the reserved capacity after a graph
   // replay (the kernels read the frame's own counts from the device plan).
   const uint64_t v_cap = c->sizes_known ? c->V : c->v_cap;
-  if (sc->count == 0 || v_cap == 0 || (c->sizes_known && c->M == 0)) return LUMEN_SUCCESS;
+  if (sc->count == 0 || v_cap == 0 || (c->sizes_known && c->M == 0)) {
+    // no splat reached a tile: every gradient is zero.  Assigning callers
+    // hand in uninitialised planes, so they are written; adding callers'
+    // planes stay as they are.
+    if (cfg->flags & LUMEN_EX_GRAD_ASSIGN) {
+      for (float* p : planes) LUMEN_TRY(fill<float>(p, 0.0f, sc->count, st));
+      LUMEN_TRY(fill<float>(grads->sh, 0.0f, sc->count * 48ull, st));
+    }
+    return LUMEN_SUCCESS;
+  }
   const uint64_t m_cap = c->sizes_known ? c->M : c->m_cap;
   LUMEN_TRY(ensure(c->partial, partial_bytes(m_cap)));
   LUMEN_TRY(ensure(c->grad_vis, grad_vis_bytes(v_cap)));

@@ -135,6 +135,10 @@ struct LumenExContext {
   hipEvent_t ev[2 * LUMEN_EX_TIME_STAGES] = {};
   // the plain frame waits on this event (recorded after the totals' copy)
   hipEvent_t ev_totals = nullptr;
+  // recorded after the per-view parameters' copy out of pinned_views; the
+  // next upload waits on it before it overwrites the staging
+  hipEvent_t ev_views = nullptr;
+  bool ev_views_pending = false;
 };
 
 // Records the start (end = false) or end of a timed stage on the stream when

@@ -62,6 +62,7 @@ __global__ void k_init_frame(uint32_t* __restrict__ bounds, SortPlan* __restrict
   bounds[1] = 0u;
   bounds[2] = 0xFFFFFFFFu;
   bounds[3] = 0u;
+  *reinterpret_cast<uint64_t*>(bounds + 4) = 0ull;  // exact instance total
   plan->passes = 0;
   plan->m = 0;
   plan->v = 0;
@@ -88,14 +89,21 @@ struct GeoCache {
 
 // Pass 1: visibility and tile count per (view, Gaussian), the geometry of
 // the visible ones, and per-block bounds of depth keys and view-local tile
-// ids (which decide the sort digits; reduced by k_reduce_bounds).
+// ids (which decide the sort digits; reduced by k_reduce_bounds).  The
+// packed word's count half is scanned modulo 2^32, so each block also sums
+// its counts exactly in 64 bits (block_inst): the frame's exact instance
+// total, against which k_plan and the host reject frames the 32-bit offsets
+// cannot address.
 __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
                                                int nviews, uint64_t* __restrict__ packed,
                                                GeoCache* __restrict__ geo,
-                                               uint4* __restrict__ block_bounds) {
+                                               uint4* __restrict__ block_bounds,
+                                               uint64_t* __restrict__ block_inst) {
   __shared__ uint32_t sb[4][4];
+  __shared__ uint64_t sinst[4];
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
+  uint64_t inst = 0ull;
   if (i < s.n) {
     const float px = s.px[i], py = s.py[i], pz = s.pz[i];
     const float s0 = s.sx[i], s1 = s.sy[i], s2 = s.sz[i];
@@ -110,6 +118,7 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
         const uint32_t cnt =
             (uint32_t)(g.max_tx - g.min_tx + 1) * (uint32_t)(g.max_ty - g.min_ty + 1);
         val = ((uint64_t)cnt << 32) | 1ull;
+        inst += cnt;
         const uint32_t dk = depth_key(g.depth);
         dmin = min(dmin, dk);
         dmax = max(dmax, dk);
@@ -129,12 +138,14 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
   dmax = wave_max_u32_all(dmax);
   tmin = wave_min_u32_all(tmin);
   tmax = wave_max_u32_all(tmax);
+  inst = wave_sum(inst);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sb[w][0] = dmin;
     sb[w][1] = dmax;
     sb[w][2] = tmin;
     sb[w][3] = tmax;
+    sinst[w] = inst;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -145,22 +156,29 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
       tmax = max(tmax, sb[k][3]);
     }
     block_bounds[blockIdx.x] = make_uint4(dmin, dmax, tmin, tmax);
+    block_inst[blockIdx.x] = sinst[0] + sinst[1] + sinst[2] + sinst[3];
   }
 }
 
-// One block folds the per-block bounds of k_count into bounds[0..3].
+// One block folds the per-block bounds of k_count into bounds[0..3] and
+// their exact instance sums into the 64-bit total at bounds + 4.
 __global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict__ block_bounds,
+                                                        const uint64_t* __restrict__ block_inst,
                                                         uint32_t nblocks,
                                                         uint32_t* __restrict__ bounds) {
   __shared__ uint32_t sb[16][4];
+  __shared__ uint64_t sinst[16];
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
+  uint64_t inst = 0ull;
   for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) {
     const uint4 x = block_bounds[b];
     dmin = min(dmin, x.x);
     dmax = max(dmax, x.y);
     tmin = min(tmin, x.z);
     tmax = max(tmax, x.w);
+    inst += block_inst[b];
   }
+  inst = wave_sum(inst);
   dmin = wave_min_u32_all(dmin);
   dmax = wave_max_u32_all(dmax);
   tmin = wave_min_u32_all(tmin);
@@ -171,6 +189,7 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict_
     sb[w][1] = dmax;
     sb[w][2] = tmin;
     sb[w][3] = tmax;
+    sinst[w] = inst;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -179,15 +198,21 @@ __global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict_
       dmax = max(dmax, sb[k][1]);
       tmin = min(tmin, sb[k][2]);
       tmax = max(tmax, sb[k][3]);
+      inst += sinst[k];
     }
     bounds[0] = dmin;
     bounds[1] = dmax;
     bounds[2] = tmin;
     bounds[3] = tmax;
+    *reinterpret_cast<uint64_t*>(bounds + 4) = inst;
   }
 }
 
 // Sort plan, clamped sizes and per-view spans from the scan and the bounds.
+// A frame whose exact instance total (bounds + 4) exceeds the key capacity,
+// or the 2^31 - 1 that int32 tile ranges address (which also covers a count
+// half that wrapped in the scan), is flagged as overflowed: k_emit then
+// writes nothing and the frame renders background.
 __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
                        const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
                        uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
@@ -199,6 +224,8 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
   if (blockIdx.x != 0) return;
   const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
+  const uint64_t exact_m = *reinterpret_cast<const uint64_t*>(bounds + 4);
+  const bool over = exact_m > m_cap || exact_m >= 0x7FFFFFFFull || (uint64_t)V > v_cap;
   // keys per thread of the sort blocks: a graph fixes it, a plain frame takes
   // it from its key count (the host decides the same from the read-back)
   const int items = force_items ? force_items : sort_items_for(M);
@@ -217,7 +244,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     plan->dbits = p.dbits;
     plan->m = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
-    plan->overflow = ((uint64_t)M > m_cap || (uint64_t)V > v_cap) ? 1u : 0u;
+    plan->overflow = over ? 1u : 0u;
     plan->items = items;
   }
   const uint32_t mc = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
@@ -305,6 +332,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
   // depth key) and the visible index
   __shared__ uint4 s_meta[256];
   __shared__ uint32_t s_vi[256];
+  if (plan->overflow) return;  // k_plan: the frame exceeds the capacities
   const int t = threadIdx.x;
   const uint32_t first = blockIdx.x * 256u;
   const uint32_t i = first + (uint32_t)t;
@@ -334,7 +362,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
     const uint64_t o_end = row + last + 1 < nv ? offsets[row + last + 1] : *totals;
     const uint32_t base = (uint32_t)(o_first >> 32), end = (uint32_t)(o_end >> 32);
     const uint32_t vend = (uint32_t)o_end;
-    if ((uint64_t)end > m_cap || (uint64_t)vend > v_cap) {  // uniform across the block
+    if (base > end || (uint64_t)end > m_cap || (uint64_t)vend > v_cap) {  // uniform across the block
       if (t == 0) atomicOr(&plan->overflow, 1u);
       continue;
     }
@@ -807,11 +835,14 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   fs.tiles = tb;
   fs.pixels = pb;
   const size_t bytes = sizeof(ViewParams) * n;
+  // the staging buffer is reused: the previous frame's copy out of it (which
+  // may still be queued behind that frame's work) must have run
+  if (c->ev_views_pending) {
+    LUMEN_HIP(hipEventSynchronize(c->ev_views));
+    c->ev_views_pending = false;
+  }
   if (c->pinned_views_cap < bytes) {
-    if (c->pinned_views) {
-      LUMEN_HIP(hipStreamSynchronize(st));
-      LUMEN_HIP(hipHostFree(c->pinned_views));
-    }
+    if (c->pinned_views) LUMEN_HIP(hipHostFree(c->pinned_views));
     c->pinned_views = nullptr;
     LUMEN_HIP(hipHostMalloc(&c->pinned_views, bytes, hipHostMallocDefault));
     c->pinned_views_cap = bytes;
@@ -821,6 +852,9 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   LUMEN_TRY(ensure(c->spans, sizeof(ViewSpan) * n));
   LUMEN_TRY(ensure(c->segs, sizeof(SortSeg) * n));
   LUMEN_HIP(hipMemcpyAsync(c->views.ptr, c->pinned_views, bytes, hipMemcpyHostToDevice, st));
+  if (!c->ev_views) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_views, hipEventDisableTiming));
+  LUMEN_HIP(hipEventRecord(c->ev_views, st));
+  c->ev_views_pending = true;
   return LUMEN_SUCCESS;
 }
 
@@ -865,9 +899,11 @@ static LumenStatus frame_buffers(LumenExContext* c, Need&& need, uint64_t N, con
   const uint64_t nv = (uint64_t)fs.nviews * N;
   LUMEN_TRY(need(c->packed, (nv ? nv : 1) * 8));
   LUMEN_TRY(need(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
-  LUMEN_TRY(need(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 16));
+  // per count block: bounds (16 B), then after all of them the exact instance sums (8 B)
+  LUMEN_TRY(need(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 24));
   LUMEN_TRY(need(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
-  LUMEN_TRY(need(c->totals, 32));  // 8 B totals, then the 16 B bounds (one read-back)
+  // 8 B totals, the 16 B bounds and the 8 B exact instance total (one read-back)
+  LUMEN_TRY(need(c->totals, 32));
   LUMEN_TRY(need(c->plan, sizeof(SortPlan)));
   LUMEN_TRY(need(c->ranges, fs.tiles * 8));
   if (save_pixels) {
@@ -905,7 +941,8 @@ static LumenStatus ensure_capacity(LumenExContext* c, uint64_t v_cap, uint64_t m
 static uint64_t graph_m_cap(uint64_t m) { return m + m / 4 + 65536; }
 
 // The frame's key bounds (depth key min, max, tile id min, max) sit right
-// after the 8-byte totals, so one copy reads both back.
+// after the 8-byte totals, followed by the exact 64-bit instance total, so
+// one 32-byte copy reads all three back.
 static uint32_t* frame_bounds(LumenExContext* c) {
   return reinterpret_cast<uint32_t*>(c->totals.as<uint8_t>() + 8);
 }
@@ -921,12 +958,13 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
   }
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
   const uint32_t nblocks = div_up(sd.n, 256);
+  uint4* bb = c->block_bounds.as<uint4>();
+  uint64_t* binst = reinterpret_cast<uint64_t*>(bb + nblocks);
   hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(256), 0, st, sd, c->views.as<ViewParams>(),
-                     (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(),
-                     c->block_bounds.as<uint4>());
+                     (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(), bb, binst);
   LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, st, c->block_bounds.as<uint4>(),
-                     nblocks, frame_bounds(c));
+  hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, st, bb, binst, nblocks,
+                     frame_bounds(c));
   LUMEN_LAUNCHED();
   const uint64_t nv = (uint64_t)fs.nviews * sd.n;
   LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
@@ -1059,6 +1097,15 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
                                 const LumenExCamera* cams, uint32_t nviews,
                                 const LumenExRenderConfig* cfg, float* out_rgb, float* out_depth,
                                 int* passes_out, hipStream_t st) {
+  // the scan's visible half counts (view, Gaussian) pairs in 32 bits
+  if ((uint64_t)nviews * sc->count >= 0xFFFFFFFFull)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "%u views x %llu Gaussians exceed 2^32 - 1", nviews,
+                (unsigned long long)sc->count);
+  // until this frame completes, the last frame's stages and sizes are gone
+  c->sizes_known = false;
+  c->kept = false;
+  c->pixels = false;
+  c->has_grad = false;
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
@@ -1072,7 +1119,8 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   bool ahead = false;  // plan and emission enqueued before the read-back
   if (sd.n > 0) {
     uint8_t* h = static_cast<uint8_t*>(c->pinned);
-    LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 24, hipMemcpyDeviceToHost, st));  // totals, bounds
+    // totals, bounds, exact instance total
+    LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 32, hipMemcpyDeviceToHost, st));
     if (!c->ev_totals) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_totals, hipEventDisableTiming));
     LUMEN_HIP(hipEventRecord(c->ev_totals, st));
     // Plan and emission go ahead with the capacity the buffers already hold,
@@ -1088,8 +1136,8 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
     uint64_t tot;
     memcpy(&tot, h, 8);
     memcpy(hb, h + 8, 16);
+    memcpy(&M, h + 24, 8);  // exact: the scan's count half is modulo 2^32
     V = tot & 0xFFFFFFFFull;
-    M = tot >> 32;
     if (M >= 0x7FFFFFFFull) {
       if (ahead) LUMEN_HIP(hipStreamSynchronize(st));
       return fail(LUMEN_EX_ERROR_CAPACITY, "%llu tile instances exceed 2^31-1",
@@ -1339,6 +1387,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   if (c->events_ready)
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->ev_totals) (void)hipEventDestroy(c->ev_totals);
+  if (c->ev_views) (void)hipEventDestroy(c->ev_views);
   for (DevBuf* b : context_buffers(c)) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);
@@ -1388,10 +1437,11 @@ LumenStatus lumen_ex_context_reserve(LumenExContext* c, const LumenExCamera* cam
   // the host staging of the per-view parameters
   const size_t vbytes = sizeof(ViewParams) * n_views;
   if (c->pinned_views_cap < vbytes) {
-    if (c->pinned_views) {
-      LUMEN_HIP(hipDeviceSynchronize());
-      LUMEN_HIP(hipHostFree(c->pinned_views));
+    if (c->ev_views_pending) {
+      LUMEN_HIP(hipEventSynchronize(c->ev_views));
+      c->ev_views_pending = false;
     }
+    if (c->pinned_views) LUMEN_HIP(hipHostFree(c->pinned_views));
     c->pinned_views = nullptr;
     c->pinned_views_cap = 0;
     LUMEN_HIP(hipHostMalloc(&c->pinned_views, vbytes, hipHostMallocDefault));

@@ -312,3 +312,40 @@ def test_assign_equals_accumulate_into_zeros(rasterizer, views):
     assert torch.equal(vs.observation_count, ref.observation_count)
     assert int((ref.observation_count == 0).sum()) > 0   # some Gaussians unseen
     say(f"[bwd] assign == accumulate into zeros ({views} view(s))")
+
+
+@pytest.mark.parametrize("views", [1, 2])
+def test_assign_writes_zeros_when_nothing_is_visible(rasterizer, views):
+    """A frame in which no splat reaches a tile (the camera looks away from
+    the cloud) has no gradient.  With LUMEN_EX_GRAD_ASSIGN the planes come
+    from torch.empty, so the backward must still write every entry: planes
+    and SH full of NaN / garbage come back all zero."""
+    import torch
+    from lumen_amd.batch import MultiViewRenderer
+    from lumen_amd.training import GaussianGradients, gradients_struct
+    scene = random_scene(5_000, 1.0, 12)
+    dev = scene.to_device()
+    cfg = RenderConfig()
+    # behind the camera: every splat fails the depth cull
+    cams = [look_at(CameraConfig(160, 120, 150.0, 150.0, 80.0, 60.0), (0.0, 0.0, 6.0 + k),
+                    target=(0.0, 0.0, 20.0)) for k in range(views)]
+    P = sum(c.config.width * c.config.height for c in cams)
+    g_rgb = torch.ones(3 * P, device="cuda")
+    planes = torch.full((11, scene.count), float("nan"), device="cuda")
+    sh = torch.full((scene.count, 48), float("nan"), device="cuda")
+    vs = GaussianGradients(scene.count)
+    g = gradients_struct(planes, sh, vs.viewspace_grad_2d, vs.observation_count)
+    if views == 1:
+        out = rasterizer.render(dev, cams[0], cfg, save_pixels=True)
+        assert out.visible_splats == 0 and out.total_splat_instances == 0
+        rasterizer.render_backward(dev, cams[0], cfg, g_rgb.view(120, 160, 3), None, g, assign=True)
+    else:
+        mv = MultiViewRenderer(0, rasterizer=rasterizer)
+        mv.render(dev, cams, cfg, save_pixels=True)
+        mv.synchronize()
+        mv.backward(dev, cams, cfg, g_rgb, None, g, assign=True)
+        mv.synchronize()
+    torch.cuda.synchronize()
+    assert bool((planes == 0).all()) and bool((sh == 0).all())
+    assert int(vs.observation_count.sum()) == 0
+    say(f"[bwd] nothing visible ({views} view(s)): assign wrote zeros")
