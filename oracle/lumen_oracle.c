@@ -154,6 +154,25 @@ void orc_quat_to_matrix(const float q[4], float out[9]) {
     q_to_matrix(a, out);
 }
 
+void orc_quat_inverse(const float q[4], float out[4]) {
+    quat a = {q[0], q[1], q[2], q[3]};
+    quat r = q_inverse(a);
+    out[0] = r.w; out[1] = r.x; out[2] = r.y; out[3] = r.z;
+}
+
+void orc_quat_mul(const float a[4], const float b[4], float out[4]) {
+    quat qa = {a[0], a[1], a[2], a[3]}, qb = {b[0], b[1], b[2], b[3]};
+    quat r = q_mul(qa, qb);
+    out[0] = r.w; out[1] = r.x; out[2] = r.y; out[3] = r.z;
+}
+
+void orc_quat_rotate_vec(const float q[4], const float v[3], float out[3]) {
+    quat a = {q[0], q[1], q[2], q[3]};
+    vec3 b = {v[0], v[1], v[2]};
+    vec3 r = q_rotate_vec(a, b);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Camera: src/scene/camera.mojo                                             */
 /* ------------------------------------------------------------------------ */

@@ -90,6 +90,9 @@ float orc_expf(float x);
 void orc_expf_array(const float *x, float *y, int64_t n);
 void orc_quat_normalized(const float q[4], float out[4]);
 void orc_quat_to_matrix(const float q[4], float out[9]);
+void orc_quat_inverse(const float q[4], float out[4]);
+void orc_quat_mul(const float a[4], const float b[4], float out[4]);
+void orc_quat_rotate_vec(const float q[4], const float v[3], float out[3]);
 void orc_world_to_cam(const orc_camera *cam, const float p[3], float out[3]);
 void orc_cam_to_screen(const orc_camera *cam, const float pc[3], float out[2]);
 void orc_cov3d(const float scale[3], const float q[4], float out[6]);

@@ -70,6 +70,8 @@ def lib():
         L.orc_expf_array.argtypes = [f32p, f32p, ctypes.c_int64]
         for name, args in {
             "orc_quat_normalized": [f32p, f32p], "orc_quat_to_matrix": [f32p, f32p],
+            "orc_quat_inverse": [f32p, f32p], "orc_quat_mul": [f32p, f32p, f32p],
+            "orc_quat_rotate_vec": [f32p, f32p, f32p],
             "orc_world_to_cam": [ctypes.POINTER(OrcCamera), f32p, f32p],
             "orc_cam_to_screen": [ctypes.POINTER(OrcCamera), f32p, f32p],
             "orc_cov3d": [f32p, f32p, f32p],

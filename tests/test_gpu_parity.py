@@ -218,6 +218,19 @@ def test_random1m_1920x1080_full_parity(rasterizer, oracle):
     say(f"[parity] 1M: V={ref['visible']} M={ref['instances']} E={ref['evaluations']}")
 
 
+@pytest.mark.parametrize("eye", [(1.3, 3.9, 2.8), (1.3, 0.7, 4.8)])
+def test_random1m_1920x1080_off_axis_camera(rasterizer, oracle, eye):
+    """The 1M-splat 1080p scene from off-axis look-at cameras whose rotations
+    are not their own inverses (2.26 and 3.00 rad): the camera transform,
+    view rotation and every later stage bit-exact against the oracle (whose
+    camera is pinned to the rotation library in tests/test_oracle_camera.py)."""
+    sc, _ = workload("random1m_1920x1080")
+    cfg = CameraConfig(1920, 1080, 1400.0, 1400.0, 960.0, 540.0, 0.01, 1000.0)
+    ref, out = compare(rasterizer, oracle, sc, look_at(cfg, eye))
+    assert ref["visible"] > 300_000
+    say(f"[parity] 1M off-axis {eye}: V={ref['visible']} M={ref['instances']}")
+
+
 # ---- sort and ranges through lumen_ex_radix_sort / lumen_ex_build_ranges --
 
 def _lcg(n, seed=123456789):
