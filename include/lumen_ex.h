@@ -263,6 +263,21 @@ LumenStatus lumen_ex_photometric_loss(LumenExContext* ctx, const float* pred, co
                                       uint32_t width, uint32_t height, float lambda_dssim,
                                       float* out, float* grad, void* stream);
 
+/* lumen_ex_photometric_loss with no context and no allocation, ever: its
+ * partial sums live in a fixed 2 MiB device array of the library (one per
+ * GPU, held until the process ends), large enough for images up to
+ * 8192 x 8192; larger images return LUMEN_ERROR_INVALID_ARGUMENT.  The
+ * images, out and grad are on the current device; the result is bit for bit
+ * lumen_ex_photometric_loss's.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under hipGraph
+ * capture, serving loops that forbid allocation).  Every call on one GPU
+ * shares the array: calls that overlap on one GPU (different streams, or
+ * different host threads) race, so serialize them. */
+LumenStatus lumen_ex_photometric_loss_noalloc(const float* pred, const float* target,
+                                              uint32_t width, uint32_t height,
+                                              float lambda_dssim, float* out, float* grad,
+                                              void* stream);
+
 /* ---- the optimizer side of a training step --------------------------------
  * Mutable Gaussian planes in the GaussianSoA layout (the scene itself, or a
  * per-Gaussian state such as Adam's moments): 11 planes of count floats and

@@ -182,6 +182,10 @@ _SIGS = {
                                                  ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                                                  ctypes.c_void_p, ctypes.c_void_p,
                                                  ctypes.c_void_p]),
+    "lumen_ex_photometric_loss_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                         ctypes.c_uint32, ctypes.c_uint32,
+                                                         ctypes.c_float, ctypes.c_void_p,
+                                                         ctypes.c_void_p, ctypes.c_void_p]),
     "lumen_ex_workspace_size": (ctypes.c_int, [ctypes.POINTER(LumenExCamera), ctypes.c_uint32,
                                                ctypes.POINTER(LumenExRenderConfig),
                                                ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,

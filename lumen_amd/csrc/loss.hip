@@ -21,6 +21,8 @@
 // dx_i, dy_i the centred values (the sums of dx and dy over a window are 0).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "context.hpp"
 #include "lumen_ex.h"
@@ -185,9 +187,90 @@ __global__ __launch_bounds__(1024) void k_loss_final(const float* __restrict__ p
   }
 }
 
+// The partial sums of lumen_ex_photometric_loss_noalloc: one array per
+// device for the life of the process, 2 floats per block of four 8x8
+// windows plus at most 2048 blocks of the remaining pixels, enough for
+// images up to 8192 x 8192 (2.02 MiB).  Host code never passes this name to
+// a kernel: it is a host-side stand-in, and the device address comes from
+// hipGetSymbolAddress (scratch_ptr below).
+constexpr size_t kLossScratchFloats = 2u * ((8192u / 8u) * (8192u / 8u) / 4u + 2048u);
+static __device__ __align__(256) float scratch[kLossScratchFloats];
+
+// Enqueues the loss of one image pair with its partial sums at `part`
+// (loss_part_bytes(width, height) bytes).
+static LumenStatus enqueue_loss(const float* pred, const float* target, uint32_t width,
+                                uint32_t height, float lambda_dssim, float* out, float* grad,
+                                float* part, hipStream_t st) {
+  LossShape sh;
+  sh.width = width;
+  sh.height = height;
+  sh.wx = width / kLossWin;
+  sh.hy = height / kLossWin;
+  sh.inv_n = 1.0f / (float)((uint64_t)width * height * 3u);
+  const uint32_t nwin = sh.wx * sh.hy;
+  sh.w_l1 = 1.0f - lambda_dssim;
+  sh.w_ssim = nwin ? lambda_dssim / (float)nwin : 0.0f;
+  const uint32_t bw = loss_window_blocks(width, height);
+  const uint32_t br = loss_rest_blocks(width, height);
+  float* pw = part;
+  float* pr = pw + 2u * bw;
+  hipLaunchKernelGGL(k_loss_windows, dim3(bw), dim3(256), 0, st, pred, target, sh, grad, pw);
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_loss_rest, dim3(br), dim3(256), 0, st, pred, target, sh, grad, pr);
+  LUMEN_LAUNCHED();
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, st, pw, bw, pr, br, sh, lambda_dssim,
+                     out);
+  LUMEN_LAUNCHED();
+  return LUMEN_SUCCESS;
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+constexpr int kMaxLossDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxLossDevices] = {};
+
+static LumenStatus scratch_ptr(int device, float** p) {
+  if (device < 0 || device >= kMaxLossDevices)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(LUMEN_EX_ERROR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                  hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<float*>(ptr);
+  }
+  *p = g_scratch_ptr[device];
+  return LUMEN_SUCCESS;
+}
+
 }  // namespace lumen
 
 using namespace lumen;
+
+extern "C" LumenStatus lumen_ex_photometric_loss_noalloc(const float* pred, const float* target,
+                                                         uint32_t width, uint32_t height,
+                                                         float lambda_dssim, float* out,
+                                                         float* grad, void* stream) {
+  clear_error();
+  if (!pred || !target || !out)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_photometric_loss_noalloc: null argument");
+  if (width == 0 || height == 0)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_photometric_loss_noalloc: empty image");
+  if (loss_part_bytes(width, height) > sizeof(scratch))
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT,
+                "lumen_ex_photometric_loss_noalloc: %ux%u needs %zu bytes of partial sums, more "
+                "than its fixed %zu (images up to 8192 x 8192)",
+                width, height, loss_part_bytes(width, height), sizeof(scratch));
+  int device = 0;
+  LUMEN_HIP(hipGetDevice(&device));
+  float* part = nullptr;
+  LUMEN_TRY(scratch_ptr(device, &part));
+  return enqueue_loss(pred, target, width, height, lambda_dssim, out, grad, part,
+                      (hipStream_t)stream);
+}
 
 extern "C" LumenStatus lumen_ex_photometric_loss(LumenExContext* c, const float* pred,
                                                  const float* target, uint32_t width,
@@ -200,26 +283,7 @@ extern "C" LumenStatus lumen_ex_photometric_loss(LumenExContext* c, const float*
     return fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_photometric_loss: empty image");
   hipStream_t st = (hipStream_t)stream;
   LUMEN_HIP(hipSetDevice(c->device));
-  LossShape sh;
-  sh.width = width;
-  sh.height = height;
-  sh.wx = width / kLossWin;
-  sh.hy = height / kLossWin;
-  sh.inv_n = 1.0f / (float)((uint64_t)width * height * 3u);
-  const uint32_t nwin = sh.wx * sh.hy;
-  sh.w_l1 = 1.0f - lambda_dssim;
-  sh.w_ssim = nwin ? lambda_dssim / (float)nwin : 0.0f;
-  const uint32_t bw = loss_window_blocks(width, height);
-  const uint32_t br = loss_rest_blocks(width, height);
   LUMEN_TRY(ensure(c->loss_part, loss_part_bytes(width, height)));
-  float* pw = c->loss_part.as<float>();
-  float* pr = pw + 2u * bw;
-  hipLaunchKernelGGL(k_loss_windows, dim3(bw), dim3(256), 0, st, pred, target, sh, grad, pw);
-  LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_loss_rest, dim3(br), dim3(256), 0, st, pred, target, sh, grad, pr);
-  LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(1024), 0, st, pw, bw, pr, br, sh, lambda_dssim,
-                     out);
-  LUMEN_LAUNCHED();
-  return LUMEN_SUCCESS;
+  return enqueue_loss(pred, target, width, height, lambda_dssim, out, grad,
+                      c->loss_part.as<float>(), st);
 }
