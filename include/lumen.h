@@ -5,8 +5,15 @@
  * field order and LP64 sizes, as the reference's c_api/include/lumen.h:29-159,
  * so that existing callers (c_api/test/test_c_api.c, the Python SDK's ctypes
  * loader python/lumen/client.py:47-65) bind to this library unchanged.
- * lumen_rasterize here runs the GPU splat pipeline (see lumen_ex.h for the
- * full-precision entry points).
+ * lumen_rasterize here runs on the GPU with one of two renderers, chosen per
+ * rasterizer with lumen_ex_rasterizer_set_mode (lumen_ex.h):
+ *   - by default the splat pipeline (anisotropic EWA footprints, SH colour
+ *     to degree 3), whose pixels differ from the reference C library's: the
+ *     C test's observable contract holds (c_api/test/test_c_api.c:127-172),
+ *     the pixel values do not;
+ *   - LUMEN_EX_RASTER_LEGACY: the reference C library's own isotropic,
+ *     DC-only renderer (c_api/src/lumen_c.c:380-575), bit for bit.
+ * See lumen_ex.h for the full-precision entry points.
  */
 #ifndef LUMEN_H
 #define LUMEN_H
@@ -127,7 +134,10 @@ LumenFrameBuffer* lumen_framebuffer_create(uint32_t width, uint32_t height);
 void lumen_framebuffer_destroy(LumenFrameBuffer* fb);
 void lumen_framebuffer_clear(LumenFrameBuffer* fb, uint8_t r, uint8_t g, uint8_t b);
 
-/* c_api/include/lumen.h:149-159 */
+/* c_api/include/lumen.h:149-159.  lumen_rasterize accumulates into the
+ * framebuffer's existing contents; its renderer is the rasterizer's mode
+ * (default: the splat pipeline, pixels differ from the reference C
+ * library's; LUMEN_EX_RASTER_LEGACY: the reference's renderer bit for bit). */
 LumenRasterizer* lumen_rasterizer_create(const LumenConfig* config);
 void lumen_rasterizer_destroy(LumenRasterizer* rasterizer);
 LumenStatus lumen_rasterizer_set_config(LumenRasterizer* rasterizer, const LumenConfig* config);

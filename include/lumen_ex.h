@@ -68,7 +68,7 @@ typedef struct {
 /* RenderConfig fields the reference reads (src/core/config.mojo:65-102). */
 typedef struct {
     uint32_t tile_width;     /* 16 */
-    uint32_t tile_height;    /* 16; tile_width*tile_height must be 64..1024, a multiple of 64 */
+    uint32_t tile_height;    /* 16; tile_width*tile_height 64, 128, 192 or 256, tile_width dividing 64 */
     float lowpass_filter;    /* 0.3 */
     int32_t max_sh_degree;   /* 0..3 */
     int32_t white_background;
@@ -383,6 +383,25 @@ LumenStatus lumen_ex_stage_times(LumenExContext* ctx, float* ms_out, int n);
 
 /* Message of the last failing lumen_ex_* call on this thread ("" if none). */
 const char* lumen_ex_last_error(void);
+
+/* ---- lumen_rasterize's renderer ------------------------------------------
+ * lumen_rasterize (lumen.h) renders with one of two algorithms:
+ *   LUMEN_EX_RASTER_PIPELINE (the default): the splat pipeline of
+ *     lumen_ex_render (anisotropic EWA footprints, SH colour up to degree 3,
+ *     16x16 front-to-back compositing), composited into the framebuffer's
+ *     8-bit colour and float alpha/depth planes.  Its pixels differ from the
+ *     reference C library's, whose renderer is a simpler model.
+ *   LUMEN_EX_RASTER_LEGACY: the reference C library's own renderer
+ *     (c_api/src/lumen_c.c:380-575) on the GPU, bit for bit: isotropic
+ *     radius sigma * max(scale) * f / z, DC-only colour, splats in ascending
+ *     depth (ties in cloud order), saturating 8-bit accumulation into the
+ *     framebuffer.  Tile size and image sides are limited to 1..65535.
+ * The mode belongs to the rasterizer handle and persists across calls.  The
+ * reference has no such switch: it only has the legacy renderer. */
+#define LUMEN_EX_RASTER_PIPELINE 0
+#define LUMEN_EX_RASTER_LEGACY 1
+LumenStatus lumen_ex_rasterizer_set_mode(LumenRasterizer* rasterizer, int mode);
+int lumen_ex_rasterizer_get_mode(const LumenRasterizer* rasterizer);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,8 @@ LUMEN_EX_KEEP_STAGES = 1
 LUMEN_EX_SAVE_PIXELS = 2
 LUMEN_EX_GRAPH = 4
 LUMEN_EX_GRAD_ASSIGN = 8
+LUMEN_EX_RASTER_PIPELINE = 0
+LUMEN_EX_RASTER_LEGACY = 1
 LUMEN_EX_ERROR_CAPACITY = -101
 
 STAGES = {"vis_to_orig": 0, "records": 1, "radius": 2, "keys_unsorted": 3, "vals_unsorted": 4,
@@ -142,6 +144,8 @@ _SIGS = {
     "lumen_rasterizer_set_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LumenConfig)]),
     "lumen_rasterize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(LumenCamera),
                                        ctypes.POINTER(LumenFrameBuffer), ctypes.POINTER(LumenStats)]),
+    "lumen_ex_rasterizer_set_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "lumen_ex_rasterizer_get_mode": (ctypes.c_int, [ctypes.c_void_p]),
     "lumen_ex_abi_version": (ctypes.c_int, []),
     "lumen_ex_last_error": (ctypes.c_char_p, []),
     "lumen_ex_context_create": (ctypes.c_void_p, [ctypes.c_int]),
@@ -259,6 +263,13 @@ def lib() -> ctypes.CDLL:
                 raise RuntimeError(
                     f"{LIB_PATH} is missing: build it with `python -m lumen_amd.build` "
                     "(there is no fallback path)")
+            # PyTorch ships its own HIP runtime under the same soname
+            # (libamdhip64.so.7).  Importing it first makes liblumen.so bind
+            # to that runtime, so that the process holds one HIP runtime
+            # whichever of the two a caller uses first; loaded the other way
+            # round, torch would find the system runtime already in place
+            # and fail to initialize its devices.
+            import torch  # noqa: F401
             handle = ctypes.CDLL(LIB_PATH)
             for name, (res, args) in _SIGS.items():
                 fn = getattr(handle, name)

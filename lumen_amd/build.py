@@ -21,7 +21,8 @@ CSRC = os.path.join(ROOT, "lumen_amd", "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(ROOT, "lumen_amd", "liblumen.so")
 
-SOURCES = ["liblumen.hip", "capi_legacy.hip", "backward.hip", "loss.hip", "optim.hip"]
+SOURCES = ["liblumen.hip", "capi_legacy.hip", "legacy.hip", "backward.hip", "loss.hip",
+           "optim.hip"]
 ARCH = os.environ.get("LUMEN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

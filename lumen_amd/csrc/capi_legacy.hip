@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "legacy.hpp"
 #include "lumen.h"
 #include "lumen_ex.h"
 
@@ -40,6 +41,8 @@ struct LumenRasterizer {
   LumenConfig config;
   LumenExContext* ctx;
   lumen::DevBuf aos, planes, rgb, depth;
+  int mode;                     // LUMEN_EX_RASTER_PIPELINE or LUMEN_EX_RASTER_LEGACY
+  lumen::LegacyBuffers legacy;  // the legacy mode's device buffers
 };
 
 namespace {
@@ -272,6 +275,7 @@ LumenRasterizer* lumen_rasterizer_create(const LumenConfig* config) {
   if (!r) return nullptr;
   r->config = config ? *config : lumen_default_config();
   r->ctx = nullptr;
+  r->mode = LUMEN_EX_RASTER_PIPELINE;
   return r;
 }
 
@@ -282,6 +286,7 @@ void lumen_rasterizer_destroy(LumenRasterizer* r) {
     lumen::release(r->planes);
     lumen::release(r->rgb);
     lumen::release(r->depth);
+    lumen::legacy_release(r->legacy);
     lumen_ex_context_destroy(r->ctx);
   }
   delete r;
@@ -293,10 +298,53 @@ LumenStatus lumen_rasterizer_set_config(LumenRasterizer* r, const LumenConfig* c
   return LUMEN_SUCCESS;
 }
 
+LumenStatus lumen_ex_rasterizer_set_mode(LumenRasterizer* r, int mode) {
+  if (!r || (mode != LUMEN_EX_RASTER_PIPELINE && mode != LUMEN_EX_RASTER_LEGACY))
+    return lumen::fail(LUMEN_ERROR_INVALID_ARGUMENT, "lumen_ex_rasterizer_set_mode: bad argument");
+  r->mode = mode;
+  return LUMEN_SUCCESS;
+}
+
+int lumen_ex_rasterizer_get_mode(const LumenRasterizer* r) {
+  return r ? r->mode : LUMEN_EX_RASTER_PIPELINE;
+}
+
+// lumen_rasterize in LUMEN_EX_RASTER_LEGACY mode: the reference C library's
+// own renderer, bit for bit (legacy.hip).
+static LumenStatus rasterize_legacy(LumenRasterizer* r, const LumenCloud* cloud,
+                                    const LumenCamera* cam, LumenFrameBuffer* fb, LumenStats* stats,
+                                    std::chrono::steady_clock::time_point t0) {
+  const uint64_t n = cloud->count;
+  LUMEN_TRY(lumen::ensure(r->aos, (n ? n : 1) * sizeof(LumenGaussian)));
+  if (n) LUMEN_HIP(hipMemcpy(r->aos.ptr, cloud->data, n * sizeof(LumenGaussian), hipMemcpyHostToDevice));
+  uint64_t st[5];
+  LUMEN_TRY(lumen::legacy_render(r->ctx, r->legacy, r->aos.as<float>(), n, r->config, *cam, fb, st));
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    stats->total_splats = st[0];
+    stats->visible_splats = st[1];
+    stats->culled_splats = st[2];
+    stats->total_tiles = st[3];
+    stats->touched_tiles = st[4];
+    stats->rasterize_time_ms = (float)ms;
+    stats->frames_per_second = ms > 0.001 ? (float)(1000.0 / ms) : 10000.0f;
+  }
+  return LUMEN_SUCCESS;
+}
+
 LumenStatus lumen_rasterize(LumenRasterizer* r, const LumenCloud* cloud, const LumenCamera* cam,
                             LumenFrameBuffer* fb, LumenStats* stats) {
   if (!r || !cloud || !cam || !fb) return LUMEN_ERROR_INVALID_ARGUMENT;
   const auto t0 = std::chrono::steady_clock::now();
+  if (r->mode == LUMEN_EX_RASTER_LEGACY) {
+    if (!r->ctx) {
+      r->ctx = lumen_ex_context_create(0);
+      if (!r->ctx) return LUMEN_ERROR_UNINITIALIZED;
+    }
+    LUMEN_HIP(hipSetDevice(0));
+    return rasterize_legacy(r, cloud, cam, fb, stats, t0);
+  }
   const uint32_t W = fb->width, H = fb->height;
   uint32_t ts = r->config.tile_size;
   if (ts * ts < 64 || ts * ts > 256) ts = 16;  // the GPU compositor needs 64..256 lanes per tile

@@ -8,7 +8,11 @@ colour trunc(255 min(1, C)), alpha = 1 - T and depth = D / alpha (1000 where
 nothing was drawn), with the reference's stats keys.  render_batch renders
 many cameras of one GPU-resident scene in one lumen_ex_render_batch frame
 (CameraTrajectory.orbit feeds it) and returns, per camera, exactly what
-render returns.  There is no CPU fallback: a missing liblumen.so raises.
+render returns.  LumenEngine(legacy=True) selects lumen_rasterize's legacy
+mode instead: the reference C library's own isotropic, DC-only renderer, bit
+for bit, whose model is the one the reference SDK's pure-Python render loop
+follows (python/lumen/client.py:67-219, there in float64).  There is no CPU
+fallback: a missing liblumen.so raises.
 """
 from __future__ import annotations
 
@@ -77,8 +81,9 @@ def _stats(total, visible, total_tiles, touched, ms) -> Dict[str, Any]:
 
 
 class LumenEngine:
-    def __init__(self, config: Optional[RasterizerConfig] = None):
+    def __init__(self, config: Optional[RasterizerConfig] = None, legacy: bool = False):
         self.config = config if config is not None else RasterizerConfig()
+        self.legacy = bool(legacy)
         self._c_lib = _lib.lib()
         self._rasterizer = None
         self._batch = None
@@ -121,6 +126,8 @@ class LumenEngine:
             self._rasterizer = L.lumen_rasterizer_create(ctypes.byref(cfg))
             if not self._rasterizer:
                 raise MemoryError("lumen_rasterizer_create failed")
+            mode = _lib.LUMEN_EX_RASTER_LEGACY if self.legacy else _lib.LUMEN_EX_RASTER_PIPELINE
+            _check("lumen_ex_rasterizer_set_mode", L.lumen_ex_rasterizer_set_mode(self._rasterizer, mode))
         else:
             _check("lumen_rasterizer_set_config",
                    L.lumen_rasterizer_set_config(self._rasterizer, ctypes.byref(cfg)))
@@ -153,6 +160,8 @@ class LumenEngine:
         t0 = time.perf_counter()
         if not cameras:
             return []
+        if self.legacy:  # the C library's renderer is one camera per call
+            return [self.render(cloud, c) for c in cameras]
         w, h = self.config.image_width, self.config.image_height
         if self._batch is None:
             self._batch = MultiViewRenderer(0)

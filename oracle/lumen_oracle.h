@@ -172,6 +172,29 @@ float orc_ssim(const float *pred, const float *target, int64_t w, int64_t h, int
 float orc_psnr(const float *pred, const float *target, int64_t n);
 float orc_combined_loss(float l1, float ssim, float lambda_dssim);
 
+/* ---- the legacy C renderer (lumen_oracle_legacy.c) --------------------- */
+/* LumenCamera (c_api/include/lumen.h:65-73): position, target, up, fov x/y
+ * in degrees, near, far (the renderer reads the config's planes instead). */
+typedef struct {
+    float pos[3], target[3], up[3];
+    float fov_x_deg, fov_y_deg, near_plane, far_plane;
+} orc_legacy_camera;
+/* the LumenConfig fields lumen_rasterize reads (c_api/include/lumen.h:39-47) */
+typedef struct {
+    float near_plane, far_plane, early_termination_alpha, gaussian_radius_sigma;
+    uint32_t tile_size;
+} orc_legacy_config;
+typedef struct {
+    uint64_t total, visible, culled, total_tiles, touched;
+} orc_legacy_stats;
+/* lumen_rasterize (c_api/src/lumen_c.c:380-575) over n AoS LumenGaussians
+ * (59 floats each) into a width x height framebuffer whose planes it reads
+ * and updates in place.  Returns -1 for a zero tile size (the C library
+ * divides by it), else 0. */
+int orc_legacy_rasterize(const float *aos, int64_t n, const orc_legacy_camera *cam,
+                         const orc_legacy_config *cfg, uint32_t width, uint32_t height,
+                         uint8_t *rgb, float *depth, float *alpha, orc_legacy_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,24 @@ class OrcResult(ctypes.Structure):
         ("ranges", i64p), ("rgb", f32p), ("depth", f32p), ("t_final", f32p), ("n_eval", i32p)]
 
 
+class OrcLegacyCamera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("target", ctypes.c_float * 3),
+                ("up", ctypes.c_float * 3), ("fov_x_deg", ctypes.c_float),
+                ("fov_y_deg", ctypes.c_float), ("near_plane", ctypes.c_float),
+                ("far_plane", ctypes.c_float)]
+
+
+class OrcLegacyConfig(ctypes.Structure):
+    _fields_ = [("near_plane", ctypes.c_float), ("far_plane", ctypes.c_float),
+                ("early_termination_alpha", ctypes.c_float),
+                ("gaussian_radius_sigma", ctypes.c_float), ("tile_size", ctypes.c_uint32)]
+
+
+class OrcLegacyStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("total", "visible", "culled", "total_tiles",
+                                               "touched")]
+
+
 _lib = None
 
 
@@ -108,6 +126,11 @@ def lib():
         L.orc_ssim.argtypes = [f32p, f32p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
         L.orc_psnr.restype = ctypes.c_float
         L.orc_psnr.argtypes = [f32p, f32p, ctypes.c_int64]
+        L.orc_legacy_rasterize.restype = ctypes.c_int
+        L.orc_legacy_rasterize.argtypes = [f32p, ctypes.c_int64, ctypes.POINTER(OrcLegacyCamera),
+                                           ctypes.POINTER(OrcLegacyConfig), ctypes.c_uint32,
+                                           ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint8), f32p,
+                                           f32p, ctypes.POINTER(OrcLegacyStats)]
         L.orc_combined_loss.restype = ctypes.c_float
         L.orc_combined_loss.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
         _lib = L
@@ -385,3 +408,32 @@ def densify_and_prune(planes, sh, vs_grad, vs_count, cfg: OrcTrainConfig, scene_
     assert m >= 0
     return (np.ascontiguousarray(op[:, :m]), osh[:m].copy(), origin[:m].copy(),
             tuple(int(x) for x in stats))
+
+
+def legacy_rasterize(aos, camera, config, rgb, depth, alpha):
+    """The reference C library's lumen_rasterize (c_api/src/lumen_c.c:380-575)
+    restated: aos (n, 59) float32 LumenGaussians; camera = dict(pos, target,
+    up, fov_x, fov_y, near, far); config = dict(near, far, eta, sigma, tile);
+    rgb (H, W, 3) uint8, depth and alpha (H, W) float32 framebuffer planes,
+    updated in place.  Returns the stats as a dict."""
+    aos = np.ascontiguousarray(aos, np.float32)
+    H, W = depth.shape
+    assert rgb.shape == (H, W, 3) and rgb.dtype == np.uint8 and rgb.flags["C_CONTIGUOUS"]
+    assert alpha.shape == (H, W) and depth.flags["C_CONTIGUOUS"] and alpha.flags["C_CONTIGUOUS"]
+    cam = OrcLegacyCamera()
+    for k in range(3):
+        cam.pos[k], cam.target[k], cam.up[k] = camera["pos"][k], camera["target"][k], camera["up"][k]
+    cam.fov_x_deg, cam.fov_y_deg = camera["fov_x"], camera["fov_y"]
+    cam.near_plane, cam.far_plane = camera["near"], camera["far"]
+    cfg = OrcLegacyConfig(config["near"], config["far"], config["eta"], config["sigma"],
+                          config["tile"])
+    st = OrcLegacyStats()
+    rc = lib().orc_legacy_rasterize(aos.ctypes.data_as(f32p) if aos.size else None,
+                                    aos.shape[0] if aos.size else 0, ctypes.byref(cam),
+                                    ctypes.byref(cfg), W, H,
+                                    rgb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                    fptr(depth), fptr(alpha), ctypes.byref(st))
+    if rc != 0:
+        raise ValueError("orc_legacy_rasterize: invalid argument")
+    return {k: int(getattr(st, k)) for k in ("total", "visible", "culled", "total_tiles",
+                                             "touched")}
