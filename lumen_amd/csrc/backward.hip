@@ -84,7 +84,9 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   __shared__ float s_part[4 / PIX][BATCH * kGradVis];  // per-wave sums, one slot per splat
   __shared__ int s_last;
   if (plan->overflow) return;
-  const uint32_t gt = blockIdx.x;
+  // each XCD walks a contiguous run of tiles: the slots of a splat covering
+  // neighbouring tiles are neighbours, and their partial lines merge in L2
+  const uint32_t gt = xcd_block(blockIdx.x, gridDim.x);
   const int vw = view_of_tile_b(views, nviews, gt);
   const ViewParams& v = views[vw];
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;

@@ -207,6 +207,19 @@ __device__ __forceinline__ T block_exclusive_scan(T x, T* s_wave, T& total) {
   return prefix + inc - x;
 }
 
+// The blocks of a launch are dealt to the 8 XCDs in turn (block i runs on
+// XCD i % 8), each XCD with its own L2.  xcd_block renumbers blocks 0..n-1 so
+// that each XCD takes a contiguous run of them: neighbouring blocks' writes
+// to shared cache lines (histogram rows of neighbouring sort blocks, the
+// ends of their digit runs, the gradient slots of a splat that covers
+// neighbouring tiles) then go through one L2, where the pieces merge before
+// they reach HBM.  A bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t n) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t q = n / kXcds, r = n % kXcds, x = i % kXcds;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i / kXcds;
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, T v, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -33,19 +33,6 @@ namespace lumen {
 
 constexpr int kSortThreads = 256;
 
-// The blocks of a launch are dealt to the 8 XCDs in turn (block i runs on
-// XCD i % 8), each XCD with its own L2.  xcd_block renumbers blocks 0..n-1 so
-// that each XCD takes a contiguous run of sort blocks: the histogram table's
-// lines (one entry per neighbouring block) and the partial lines at the ends
-// of neighbouring blocks' digit runs in the output are then written through
-// one L2, where the pieces merge before they go to HBM.  A bijection of
-// [0, n); the order of the keys does not depend on it.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t i, uint32_t n) {
-  constexpr uint32_t kXcds = 8;
-  const uint32_t q = n / kXcds, r = n % kXcds, x = i % kXcds;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i / kXcds;
-}
-
 // The segment and segment-local index of the sort block that grid block i
 // runs; false for a surplus block of a capacity-sized grid.  Segments
 // without keys have no blocks (the search takes the last segment starting at
