@@ -238,12 +238,14 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
             }
           }
         }
-        const float i01 = 0.5f * a.w;
-        const float t[kGradVis] = {-dpm * (a.z * dx + i01 * dy), -dpm * (i01 * dx + b.x * dy),
-                                   dpm * 0.5f * dx * dx,         dpm * dx * dy,
-                                   dpm * 0.5f * dy * dy,         g5,
-                                   gC0[p] * wm,                  gC1[p] * wm,
-                                   gC2[p] * wm,                  gD[p] * wm};
+        // the power's gradient as moments of the pixel offset: sum dpm dx,
+        // dpm dy, dpm dx^2, dpm dx dy, dpm dy^2 over the wave's pixels; the
+        // mean and inverse-covariance gradients are formed from the sums once
+        // per (splat, tile) (slot_gradients)
+        const float m0 = dpm * dx, m1 = dpm * dy;
+        const float t[kGradVis] = {m0,          m1,          m0 * dx,     m0 * dy,
+                                   m1 * dy,     g5,          gC0[p] * wm, gC1[p] * wm,
+                                   gC2[p] * wm, gD[p] * wm};
 #pragma unroll
         for (int k = 0; k < kGradVis; ++k) gv[k] = p == 0 ? t[k] : gv[k] + t[k];
       }
@@ -260,13 +262,26 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       }
     }
     __syncthreads();
-    for (int e = t; e < n * kGradVis; e += nthreads) {
-      float x = 0.0f;
-      for (int w = 0; w < nwaves; ++w) {
+    for (int e = t; e < n * kGradVis; e += nthreads) {  // the waves' sums, into wave 0's slot
+      float x = s_part[0][e];
+      for (int w = 1; w < nwaves; ++w) {
         x += s_part[w][e];
         s_part[w][e] = 0.0f;
       }
-      partial[(size_t)s_dst[e / kGradVis] * kGradVis + e % kGradVis] = x;
+      s_part[0][e] = x;
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += nthreads) {
+      float* m = &s_part[0][j * kGradVis];
+      const float i00 = s0[j].z, i01 = 0.5f * s0[j].w, i11 = s1[j].x;
+      // dL/d(mean) = -Sigma^-1 sum dpm d, dL/d(i00, i01, i11) = (1/2, 1, 1/2) second moments
+      const float g[kGradVis] = {-(i00 * m[0] + i01 * m[1]), -(i01 * m[0] + i11 * m[1]),
+                                 0.5f * m[2], m[3], 0.5f * m[4], m[5], m[6], m[7], m[8], m[9]};
+      float2* d2 = reinterpret_cast<float2*>(partial + (size_t)s_dst[j] * kGradVis);
+#pragma unroll
+      for (int c = 0; c < kGradVis / 2; ++c) d2[c] = make_float2(g[2 * c], g[2 * c + 1]);
+#pragma unroll
+      for (int c = 0; c < kGradVis; ++c) m[c] = 0.0f;
     }
     __syncthreads();
   }

@@ -204,7 +204,7 @@ def test_photometric_loss_vs_float64_reference(H, W):
     ref.backward()
     assert float(l1) == pytest.approx(float(l1_ref(a.double(), b.double())), rel=2e-6)
     assert float(ssim) == pytest.approx(float(ssim_ref(a.double(), b.double())), rel=2e-5, abs=1e-6)
-    assert float(loss.detach()) == pytest.approx(float(ref), rel=2e-5)
+    assert float(loss.detach()) == pytest.approx(float(ref.detach()), rel=2e-5)
     got = pa.grad.cpu().double()
     err = float((got - ra.grad).norm() / ra.grad.norm())
     say(f"[loss] {H}x{W} rel grad err {err:.2e}")
