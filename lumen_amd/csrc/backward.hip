@@ -92,7 +92,16 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int nthreads = v.tile_w * v.tile_h / PIX;
   const int nwaves = nthreads >> 6;
-  const int rows = 64 * PIX / v.tile_w;
+  static_assert(PIX == 1, "one pixel per lane");
+  // each wave covers a qw x qh block of the tile (8 x 8 in 16 x 16 tiles):
+  // squarer than full-width strips, so fewer (splat, wave) pairs meet
+  int qh = v.tile_h < 8 ? v.tile_h : 8;
+  int qw = 64 / qh;
+  if (qw > v.tile_w) {
+    qw = v.tile_w;
+    qh = 64 / qw;
+  }
+  const int nwx = v.tile_w / qw;
   const int tile = (int)(gt - v.tile_base);
   const int2 rg = ranges[gt];
   if (rg.x == -1 || rg.x >= rg.y) return;  // uniform across the block
@@ -101,8 +110,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wv = t >> 6;
-  const int px = tx * v.tile_w + t % v.tile_w;
-  const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
+  const int px = tx * v.tile_w + (wv % nwx) * qw + lane % qw;
+  const int py0 = ty * v.tile_h + (wv / nwx) * qh + lane / qw;
   for (int e = t; e < (4 / PIX) * BATCH * kGradVis; e += nthreads) (&s_part[0][0])[e] = 0.0f;
   if (t == 0) s_last = rg.x - 1;
   __syncthreads();
@@ -167,8 +176,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       s0[j] = a;                                // x, y, i00, 2 i01
       s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
       s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-      s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                      tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
+      s_mask[j] = (uint8_t)block_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                      tx * v.tile_w, ty * v.tile_h, qw, qh, nwx, nwaves);
     }
     __syncthreads();
     // this wave's list: splats up to its pixels' last evaluated one whose box

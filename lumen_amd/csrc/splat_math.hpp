@@ -415,4 +415,28 @@ __device__ __forceinline__ uint32_t strip_mask(float sx, float sy, float hx, flo
   return m;
 }
 
+// strip_mask for waves that cover qw x qh blocks of the tile (wave w: block
+// column w % nwx, row w / nwx): bit w set unless the splat's box misses the
+// block's pixel centres or, for conics with condition number up to 1e4, its
+// power's minimum over them exceeds 1.02 lim + 1e-3 (the same margins).
+__device__ __forceinline__ uint32_t block_mask(float sx, float sy, float hx, float hy, float i00,
+                                               float i01, float i11, float lim, int tile_x0,
+                                               int tile_y0, int qw, int qh, int nwx, int nwaves) {
+  const float det = i00 * i11 - i01 * i01;
+  const float tr = i00 + i11;
+  const bool exact = i00 > 0.0f && i11 > 0.0f && det > 0.0f && tr * tr <= 1.0e4f * det &&
+                     sx == sx && sy == sy && lim == lim;
+  const float cut = 1.02f * lim + 1e-3f;
+  uint32_t m = 0u;
+  for (int w = 0; w < nwaves; ++w) {
+    const int bx = tile_x0 + (w % nwx) * qw, by = tile_y0 + (w / nwx) * qh;
+    const float x0 = (float)bx + 0.5f, x1 = (float)(bx + qw) - 0.5f;
+    const float y0 = (float)by + 0.5f, y1 = (float)(by + qh) - 0.5f;
+    if (sx + hx < x0 || sx - hx > x1 || sy + hy < y0 || sy - hy > y1) continue;
+    if (exact && rect_min_power(sx, sy, i00, i01, i11, x0, x1, y0, y1) > cut) continue;
+    m |= 1u << w;
+  }
+  return m;
+}
+
 }  // namespace lumen
