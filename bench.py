@@ -42,13 +42,16 @@ stage with the largest share; its algorithmic work per launch is:
              index, radius and key metadata written) + 12 B / key
   sort       8 B / key (one histogram read) + 24 B / key per digit pass
   ranges     8 B / key + 8 B / tile
-  composite  24 FP32 operations per (pixel, splat) evaluation -- VALU bound;
-             its bytes are 52 B / key + 8 B / tile + 16 B / pixel
+  composite  30 FP32 operations per (pixel, splat) evaluation of the
+             reference's compositor (E of them per frame, counted by the
+             oracle), the scope table's fixed charge -- VALU bound; its bytes
+             are 52 B / key + 8 B / tile + 16 B / pixel
 `traffic` is the HBM bytes per launch rocprofv3 counted for that kernel
 (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy of
 known size), read from the committed summary under profiles/ when present.
 CPU baseline: the oracle (scalar C restatement of the reference path, one
-thread) rendering one frame of the headline workload on this host.
+thread) rendering the headline workload on this host, 1 warm-up and 5 timed
+frames, reported as 1 / mean with the mean and median.
 """
 from __future__ import annotations
 
@@ -66,7 +69,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-COMPOSITE_FLOPS_PER_EVAL = 24
+COMPOSITE_FLOPS_PER_EVAL = 30
 METRIC = "1M-splat 1080p frames/s (fwd, fwd+bwd); p50/p99 request latency"
 KERNELS = {"composite": "k_composite", "sort": "k_sort_hist+k_sort_scatter", "emit": "k_emit",
            "count": "k_count", "ranges": "k_ranges"}
