@@ -414,9 +414,9 @@ def serve_leg(args):
         return done, time.perf_counter() - t0
 
     # warm every scene and resolution once (allocations, first launches)
-    def server(slots, views):
+    def server(slots, views, reserved=True):
         return RenderServer(scenes, hbm_budget=budget, slots=slots, max_views=views,
-                            keep_outputs=False, target_fn=target_fn)
+                            keep_outputs=False, target_fn=target_fn, reserved=reserved)
 
     budget = 48 << 30
     srv = server(3, 16)  # warm every scene and resolution once (allocations, first launches)
@@ -426,10 +426,12 @@ def serve_leg(args):
     done, wall = serve(srv, make(31, 1e9), False)
     sat = latency_summary(done, wall)
     sat_frames = srv.frames - frames0
+    sat_replans = srv.replans
     srv.close()
     torch.cuda.empty_cache()
     log("serve: saturation, one request at a time (plain per-view)")
-    srv = server(1, 1)
+    srv = server(1, 1, reserved=False)
+    serve(srv, make(30, 1e9)[: 3 * len(names)], False)
     done_p, wall_p = serve(srv, make(31, 1e9), False)
     sat_p = latency_summary(done_p, wall_p)
     srv.close()
@@ -439,6 +441,7 @@ def serve_leg(args):
     srv = server(3, 16)
     done_r, wall_r = serve(srv, make(32, rate), True)
     rt = latency_summary(done_r, wall_r)
+    rt_replans = srv.replans
     srv.close()
     res = {
         "workload": "serve8_mixed", "scenes": {k: v.count for k, v in scenes.items()},
@@ -450,6 +453,8 @@ def serve_leg(args):
         "poisson_rate_rps": rate, "p50_ms": rt["p50_ms"], "p99_ms": rt["p99_ms"],
         "mean_ms": rt["mean_ms"], "throughput_rps": rt["throughput_rps"],
         "slots": 3, "hbm_budget_gib": budget >> 30,
+        "frames": "reserved (no host read-back) after the first frame of each (scene, resolution)",
+        "replanned_frames": {"saturation": sat_replans, "poisson": rt_replans},
     }
     del scenes
     torch.cuda.empty_cache()

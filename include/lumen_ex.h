@@ -146,6 +146,14 @@ LumenStatus lumen_ex_render(LumenExContext* ctx, const LumenExScene* scene,
                                non-default stream) and replay it while the scene,
                                cameras, config, outputs and stream are unchanged; no
                                host round trip per frame */
+#define LUMEN_EX_RESERVED 16u /* enqueue the whole frame with the visible-splat and key
+                                 capacities the context already holds (reserved with
+                                 lumen_ex_context_reserve, or grown by earlier frames)
+                                 and return at once: no host round trip, any cameras.
+                                 A frame that exceeds them renders background and
+                                 lumen_ex_frame_status reports LUMEN_EX_ERROR_CAPACITY;
+                                 on a context with no capacity yet the frame renders
+                                 plainly (and sizes it).  The backward works after it. */
 
 typedef struct {
     uint32_t vis_base;   /* first visible index of the view in the frame's arrays */

@@ -101,6 +101,7 @@ LUMEN_EX_KEEP_STAGES = 1
 LUMEN_EX_SAVE_PIXELS = 2
 LUMEN_EX_GRAPH = 4
 LUMEN_EX_GRAD_ASSIGN = 8
+LUMEN_EX_RESERVED = 16
 LUMEN_EX_RASTER_PIPELINE = 0
 LUMEN_EX_RASTER_LEGACY = 1
 LUMEN_EX_ERROR_CAPACITY = -101

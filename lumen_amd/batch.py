@@ -93,15 +93,19 @@ class MultiViewRenderer:
                config: Optional[RenderConfig] = None, graph: bool = False,
                save_pixels: bool = False, want_stats: bool = False,
                out: Optional[Tuple[torch.Tensor, torch.Tensor]] = None,
-               keep_stages: bool = False) -> BatchOutput:
+               keep_stages: bool = False, reserved: bool = False) -> BatchOutput:
         """Enqueue one frame on self.stream (the caller's current stream is
-        made to wait for the scene first).  want_stats synchronizes."""
+        made to wait for the scene first).  want_stats synchronizes.
+        reserved=True enqueues it with the capacities the context holds and
+        returns without any host round trip (LUMEN_EX_RESERVED); status()
+        then tells whether it fitted."""
         config = config or RenderConfig()
         n = len(cameras)
         cams = (_lib.LumenExCamera * n)(*[camera_struct(c) for c in cameras])
         flags = (_lib.LUMEN_EX_GRAPH if graph else 0) | \
             (_lib.LUMEN_EX_SAVE_PIXELS if save_pixels else 0) | \
-            (_lib.LUMEN_EX_KEEP_STAGES if keep_stages else 0)
+            (_lib.LUMEN_EX_KEEP_STAGES if keep_stages else 0) | \
+            (_lib.LUMEN_EX_RESERVED if reserved else 0)
         cfg = config.ex_struct(flags)
         if out is not None:
             rgb, depth = out
@@ -157,7 +161,7 @@ class RasterizeViews(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, planes, sh, cameras, config, renderer: MultiViewRenderer, graph: bool,
-                viewspace):
+                viewspace, reserved: bool = False):
         # an unused output's gradient arrives as None, not as a zero-filled image
         ctx.set_materialize_grads(False)
         scene = DeviceScene(planes.detach(), sh.detach())
@@ -169,7 +173,8 @@ class RasterizeViews(torch.autograd.Function):
             P = frame_pixels(cameras)
             rgb = torch.empty(3 * P, dtype=torch.float32, device=planes.device)
             depth = torch.empty(P, dtype=torch.float32, device=planes.device)
-        renderer.render(scene, cameras, config, graph=graph, save_pixels=True, out=(rgb, depth))
+        renderer.render(scene, cameras, config, graph=graph, save_pixels=True, out=(rgb, depth),
+                        reserved=reserved)
         torch.cuda.current_stream(planes.device).wait_stream(renderer.stream)
         if graph:
             # fresh tensor objects over the renderer's fixed planes, so that the
@@ -179,6 +184,7 @@ class RasterizeViews(torch.autograd.Function):
         ctx.viewspace = viewspace
         ctx.frame = renderer.rasterizer.frame_id
         ctx.graph = graph
+        ctx.reserved = reserved
         # outputs go through save_for_backward: kept on ctx directly they would
         # form a cycle (output -> grad_fn -> ctx -> output) that is never freed
         ctx.save_for_backward(rgb, depth)
@@ -191,17 +197,18 @@ class RasterizeViews(torch.autograd.Function):
         out_rgb, out_depth = ctx.saved_tensors
         if R.rasterizer.frame_id != ctx.frame:
             R.render(ctx.scene, ctx.cameras, ctx.config, graph=ctx.graph, save_pixels=True,
-                     out=(out_rgb, out_depth))
+                     out=(out_rgb, out_depth), reserved=ctx.reserved)
         planes, sh, gs = new_gradients(ctx.scene.count, ctx.scene.device, ctx.viewspace)
         g_rgb = grad_rgb.contiguous() if grad_rgb is not None else torch.zeros_like(out_rgb)
         g_d = grad_depth.contiguous() if grad_depth is not None else None
         R.backward(ctx.scene, ctx.cameras, ctx.config, g_rgb, g_d, gs, assign=True)
-        return planes, sh, None, None, None, None, None
+        return planes, sh, None, None, None, None, None, None
 
 
-def rasterize_views(planes, sh, cameras, config=None, renderer=None, graph=False, viewspace=None):
+def rasterize_views(planes, sh, cameras, config=None, renderer=None, graph=False, viewspace=None,
+                    reserved=False):
     return RasterizeViews.apply(planes, sh, list(cameras), config or RenderConfig(),
-                                renderer or MultiViewRenderer(), graph, viewspace)
+                                renderer or MultiViewRenderer(), graph, viewspace, reserved)
 
 
 # ---- autotuner ----------------------------------------------------------------

@@ -56,6 +56,7 @@ __device__ __forceinline__ int view_of_tile_b(const ViewParams* __restrict__ vie
 // splat whose slots contain the chunk's first slot.
 __global__ void k_chunk_seg(const SortPlan* __restrict__ plan, const uint4* __restrict__ vmeta,
                             uint32_t* __restrict__ chunk_seg) {
+  if (plan->overflow) return;  // the frame was not rendered: its metadata is stale
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t V = plan->v, M = plan->m;

@@ -1177,6 +1177,50 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   return LUMEN_SUCCESS;
 }
 
+// The reserved frame: the whole frame enqueued with the capacities the
+// buffers hold, the sizes and the sort plan left on the device (as a graph
+// replay does, without capture): the host returns at once.
+static LumenStatus render_reserved(LumenExContext* c, const LumenExScene* sc,
+                                   const LumenExCamera* cams, uint32_t nviews,
+                                   const LumenExRenderConfig* cfg, float* out_rgb,
+                                   float* out_depth, hipStream_t st) {
+  uint64_t v_now = 0, m_now = 0;
+  current_capacity(c, v_now, m_now);
+  if (v_now == 0 || m_now == 0)  // nothing sized yet: a plain frame sizes the buffers
+    return render_plain(c, sc, cams, nviews, cfg, out_rgb, out_depth, nullptr, st);
+  if ((uint64_t)nviews * sc->count >= 0xFFFFFFFFull)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "%u views x %llu Gaussians exceed 2^32 - 1", nviews,
+                (unsigned long long)sc->count);
+  c->sizes_known = false;
+  c->kept = false;
+  c->pixels = false;
+  c->has_grad = false;
+  FrameShape fs;
+  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+  LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
+  const int items = sort_items_for(m_now);
+  LUMEN_TRY(ensure(c->hist, sort_hist_len(m_now, (int)nviews, items) * sizeof(uint32_t)));
+  LUMEN_TRY(ensure(c->hist_part, sort_part_bytes(m_now, (int)nviews, items)));
+  const SceneDev sd = scene_dev(sc);
+  for (bool& r : c->stage_ran) r = false;
+  LumenExRenderConfig pc = *cfg;
+  pc.flags &= ~LUMEN_EX_KEEP_STAGES;
+  LUMEN_TRY(enqueue_count(c, sd, fs, st));
+  LUMEN_TRY(enqueue_rest(c, sd, fs, &pc, v_now, m_now, max_sort_slots(c, fs), 3, false, out_rgb,
+                         out_depth, st));
+  c->T = fs.tiles;
+  c->P = fs.pixels;
+  c->N = sc->count;
+  c->nviews = nviews;
+  c->tile_w = cfg->tile_width;
+  c->tile_h = cfg->tile_height;
+  c->v_cap = v_now;
+  c->m_cap = m_now;
+  c->pixels = save;
+  return LUMEN_SUCCESS;
+}
+
 static void graph_release(GraphCache& g) {
   if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -1500,7 +1544,10 @@ LumenStatus lumen_ex_render_batch(LumenExContext* c, const LumenExScene* sc,
     LUMEN_TRY(render_graph(c, sc, cams, n_views, cfg, out_rgb, out_depth, st));
   } else {
     if (c->graph.valid) graph_release(c->graph);
-    LUMEN_TRY(render_plain(c, sc, cams, n_views, cfg, out_rgb, out_depth, nullptr, st));
+    if (cfg->flags & LUMEN_EX_RESERVED)
+      LUMEN_TRY(render_reserved(c, sc, cams, n_views, cfg, out_rgb, out_depth, st));
+    else
+      LUMEN_TRY(render_plain(c, sc, cams, n_views, cfg, out_rgb, out_depth, nullptr, st));
   }
   if (out_stats) return lumen_ex_frame_status(c, out_stats, n_views, stream);
   return LUMEN_SUCCESS;

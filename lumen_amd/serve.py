@@ -9,14 +9,27 @@ arrived, a frame:
     (lumen_ex_render_batch; resolutions may differ), up to max_views;
   - a forward+backward request runs as a frame of its own (its gradient is
     its own), loss 0.8 L1 + 0.2 (1 - SSIM) against the request's target.
-A frame is admitted only while the sum of the admitted frames' memory
-estimates stays within the HBM budget; requests that do not fit wait.  When
-a frame's completion event fires its memory and slot are released, and every
-request in it records its completion time.  Latency = completion - arrival.
+Admission is by device memory: a frame's workspace is what
+lumen_ex_workspace_size says its shape needs (cameras, scene size, the tile
+instances expected, backward or not), and a frame is admitted only while the
+admitted frames' workspaces fit the HBM budget.  The instances a view of a
+scene at a resolution produces are learned from the frames served (the
+largest seen, with a quarter of headroom).
+
+Frames do not block the scheduler.  A frame of a known shape is enqueued as
+a reserved frame (LUMEN_EX_RESERVED): the slot's context is reserved for the
+frame's workspace (allocating only when it grows) and the whole frame is
+enqueued with no host round trip, forward+backward included.  The first frame
+of an unknown (scene, resolution) runs plainly once, which measures it.  When
+a frame's completion event fires its workspace and slot are released, every
+request in it records its completion time (latency = completion - arrival),
+and its per-view statistics are read; a reserved frame that exceeded its
+capacity (more instances than expected: its images are background) is
+rendered again plainly, and the estimate grows.
 
 Each view of a batched frame is bit-identical to rendering that camera alone
-(lumen_ex_render_batch guarantees it), so a request's result does not depend
-on what it was batched or overlapped with; tests check this.
+(lumen_ex_render_batch guarantees it, reserved or not), so a request's result
+does not depend on what it was batched or overlapped with; tests check this.
 """
 from __future__ import annotations
 
@@ -29,9 +42,10 @@ from typing import Callable, Dict, List, Optional, Sequence
 import numpy as np
 import torch
 
+from . import _lib
 from .batch import MultiViewRenderer
 from .camera import PinholeCamera
-from .raster import Rasterizer, RenderConfig
+from .raster import Rasterizer, RenderConfig, workspace_size
 from .scene import DeviceScene
 
 
@@ -67,34 +81,20 @@ class _Frame:
     requests: List[Request]
     mem: int
     event: torch.cuda.Event
+    reserved: bool = False
     keep: list = field(default_factory=list)
 
 
-def frame_memory_estimate(scene: DeviceScene, cams: Sequence[PinholeCamera],
-                          backward: bool) -> int:
-    """Upper-bound style estimate of the device memory a frame needs beyond
-    the resident scene, from the library's per-frame arrays: per (view,
-    Gaussian) 112 B (scan 8, geometry cache 32, record 48, meta 16, index and
-    radius 8) taking every Gaussian as visible; per key 24 B (double-buffered
-    8 B key + 4 B value) with keys per visible splat taken as
-    max(2, 8 (W / 1920)^2) (8 is what 1M Gaussians at 1080p measure); per
-    pixel 24 B of outputs and saved state.  A backward adds the gradient
-    planes (236 B per Gaussian), 40 B per key and 40 B per visible splat."""
-    n = scene.count
-    v = len(cams) * n
-    pix = sum(c.config.width * c.config.height for c in cams)
-    m = sum(n * max(2.0, 8.0 * (c.config.width / 1920.0) ** 2) for c in cams)
-    mem = 112 * v + 24 * m + 24 * pix
-    if backward:
-        mem += 236 * n + 40 * m + 40 * v
-    return int(mem)
+def _view_key(scene: str, cam: PinholeCamera):
+    return scene, cam.config.width, cam.config.height
 
 
 class RenderServer:
     def __init__(self, scenes: Dict[str, DeviceScene], hbm_budget: int = 32 << 30,
                  slots: int = 3, max_views: int = 16, config: Optional[RenderConfig] = None,
                  device: int = 0, keep_outputs: bool = True,
-                 target_fn: Optional[Callable[[Request], torch.Tensor]] = None):
+                 target_fn: Optional[Callable[[Request], torch.Tensor]] = None,
+                 reserved: bool = True):
         self.scenes = scenes
         self.target_fn = target_fn  # makes a fwd+bwd request's target when it has none
         self.keep_outputs = keep_outputs  # False: drop images/gradients once a request is done
@@ -102,6 +102,7 @@ class RenderServer:
         self.config = config or RenderConfig()
         self.max_views = max_views
         self.device = device
+        self.reserved = reserved    # False: every frame plain (reads its sizes back)
         self.slots = [MultiViewRenderer(device) for _ in range(slots)]
         self.free = list(range(slots))
         self.inflight: List[_Frame] = []
@@ -110,6 +111,33 @@ class RenderServer:
         self.completed: List[Request] = []
         self.t0 = 0.0
         self.frames = 0
+        self.replans = 0            # reserved frames rendered again (capacity exceeded)
+        self.instances: Dict[tuple, int] = {}  # largest instances seen per (scene, W, H) view
+
+    # -- memory -------------------------------------------------------------
+    def _expected_instances(self, scene: str, cams: Sequence[PinholeCamera]) -> Optional[int]:
+        """Tile instances a frame of these views may produce (None: a view of
+        an unseen shape)."""
+        total = 0
+        for c in cams:
+            m = self.instances.get(_view_key(scene, c))
+            if m is None:
+                return None
+            total += m
+        return total + total // 4 + 65536
+
+    def frame_memory(self, scene: str, cams: Sequence[PinholeCamera], backward: bool) -> int:
+        """The device workspace of a frame (lumen_ex_workspace_size).  For a
+        view of an unseen shape the instances are taken as 8 per Gaussian at
+        1080p, scaled with the image area (the frame that measures it is
+        plain, sized exactly)."""
+        n = self.scenes[scene].count
+        m = self._expected_instances(scene, cams)
+        if m is None:
+            m = int(sum(n * max(2.0, 8.0 * c.config.width * c.config.height / (1920.0 * 1080.0))
+                        for c in cams))
+        return workspace_size(n, list(cams), self.config, max_instances=m, backward=backward,
+                              save_pixels=backward)
 
     # -- scheduling ---------------------------------------------------------
     def _now(self) -> float:
@@ -118,14 +146,12 @@ class RenderServer:
     def _pick_frame(self) -> Optional[List[Request]]:
         """The oldest waiting request and, if it is a forward request, the other
         waiting forward requests of its scene (arrival order) while the frame's
-        memory estimate stays within one slot's share of the budget (a slot's
-        buffers keep the size of the largest frame it ran)."""
+        workspace stays within one slot's share of the budget."""
         if not self.queue:
             return None
         head = self.queue[0]
         if head.backward:
             return [head]
-        scene = self.scenes[head.scene]
         cap = self.budget // len(self.slots)
         group = [head]
         for r in self.queue:
@@ -133,7 +159,7 @@ class RenderServer:
                 break
             if r is head or r.backward or r.scene != head.scene:
                 continue
-            if frame_memory_estimate(scene, [g.camera for g in group] + [r.camera], False) > cap:
+            if self.frame_memory(head.scene, [g.camera for g in group] + [r.camera], False) > cap:
                 continue
             group.append(r)
         return group
@@ -144,8 +170,7 @@ class RenderServer:
         group = self._pick_frame()
         if not group:
             return False
-        scene = self.scenes[group[0].scene]
-        mem = frame_memory_estimate(scene, [r.camera for r in group], group[0].backward)
+        mem = self.frame_memory(group[0].scene, [r.camera for r in group], group[0].backward)
         if self.mem_in_use + mem > self.budget and self.inflight:
             return False  # wait for memory; a lone frame is always admitted
         for r in group:
@@ -154,16 +179,22 @@ class RenderServer:
         now = self._now()
         for r in group:
             r.admitted = now
-        frame = self._launch(slot, scene, group)
+        frame = self._launch(slot, group)
         frame.mem = mem
         self.mem_in_use += mem
         self.inflight.append(frame)
         self.frames += 1
         return True
 
-    def _launch(self, slot: int, scene: DeviceScene, group: List[Request]) -> _Frame:
-        R = self.slots[slot]
+    def _run_frame(self, R: MultiViewRenderer, group: List[Request], reserved: bool) -> None:
+        """Enqueues the frame's work on the slot's stream (reserved: with no
+        host round trip)."""
+        scene = self.scenes[group[0].scene]
         cams = [r.camera for r in group]
+        if reserved:
+            m = self._expected_instances(group[0].scene, cams)
+            R.rasterizer.reserve(scene.count, cams, self.config, max_instances=m,
+                                 backward=group[0].backward, save_pixels=group[0].backward)
         if group[0].backward:
             r = group[0]
             from .batch import rasterize_views
@@ -175,7 +206,7 @@ class RenderServer:
                     r.target = self.target_fn(r)
                 planes = scene.planes.detach().requires_grad_(True)
                 sh = scene.sh.detach().requires_grad_(True)
-                rgb, depth = rasterize_views(planes, sh, cams, self.config, R)
+                rgb, depth = rasterize_views(planes, sh, cams, self.config, R, reserved=reserved)
                 H, W = r.camera.config.height, r.camera.config.width
                 img = rgb.view(H, W, 3)
                 loss = combined_loss(img, r.target, rasterizer=R.rasterizer)
@@ -183,20 +214,43 @@ class RenderServer:
                 r.rgb, r.depth = img.detach(), depth.detach().view(H, W)
                 r.grad_planes, r.grad_sh = planes.grad, sh.grad
                 r.loss = loss.detach()
-                ev = torch.cuda.Event()
-                ev.record(R.stream)
-            return _Frame(slot, group, 0, ev)
-        out = R.render(scene, cams, self.config)  # outputs allocated for this frame
-        ev = torch.cuda.Event()
-        ev.record(R.stream)
+            return
+        out = R.render(scene, cams, self.config, reserved=reserved)  # outputs for this frame
         for r, rgb, dep in zip(group, out.views_rgb, out.views_depth):
             r.rgb, r.depth = rgb, dep
-        return _Frame(slot, group, 0, ev)
+
+    def _launch(self, slot: int, group: List[Request]) -> _Frame:
+        R = self.slots[slot]
+        known = self._expected_instances(group[0].scene, [r.camera for r in group]) is not None
+        reserved = self.reserved and known
+        self._run_frame(R, group, reserved)
+        ev = torch.cuda.Event()
+        ev.record(R.stream)
+        return _Frame(slot, group, 0, ev, reserved)
+
+    def _finish(self, f: _Frame) -> None:
+        """A completed frame's per-view statistics; a reserved frame that
+        exceeded its capacity is rendered again, plainly."""
+        R = self.slots[f.slot]
+        n = len(f.requests)
+        try:
+            stats = R.status(n)
+        except _lib.LumenError as e:
+            if e.status != _lib.LUMEN_EX_ERROR_CAPACITY:
+                raise
+            self.replans += 1
+            self._run_frame(R, f.requests, False)
+            R.synchronize()
+            stats = R.status(n)
+        for r, st in zip(f.requests, stats):
+            k = _view_key(r.scene, r.camera)
+            self.instances[k] = max(self.instances.get(k, 0), st.total_splat_instances)
 
     def _retire(self) -> int:
         n = 0
         for f in list(self.inflight):
             if f.event.query():
+                self._finish(f)
                 now = self._now()
                 for r in f.requests:
                     r.done = now

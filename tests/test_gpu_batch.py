@@ -272,3 +272,57 @@ def test_batch_wide_keys_and_many_views(renderer, rasterizer):
             tx, ty = cfg.tiles(cam.config.width, cam.config.height)
             tile_base += tx * ty
         say(f"[batch] {len(cams)} views, {M} keys: per-view images and keys match")
+
+
+def test_reserved_frames_equal_plain_and_report_overflow(renderer):
+    """LUMEN_EX_RESERVED: a frame enqueued with the reserved capacities and no
+    host round trip equals the plain frame bit for bit (and its backward
+    works); a frame that outgrows the capacity reports LUMEN_EX_ERROR_CAPACITY
+    and renders background, and its backward writes zero gradients."""
+    import torch
+    from lumen_amd._lib import LumenError
+    from lumen_amd.batch import MultiViewRenderer, rasterize_views
+    scene = random_scene(80_000, 1.0, 88)
+    dev = scene.to_device()
+    cams = orbit(CameraConfig(400, 300, 380.0, 380.0, 200.0, 150.0), 6, 5.0)
+    cfg = RenderConfig()
+    plain = renderer.render(dev, cams, cfg, want_stats=True)
+    renderer.synchronize()
+    M = sum(s.total_splat_instances for s in plain.stats)
+    R = MultiViewRenderer(0)
+    R.rasterizer.reserve(scene.count, cams, cfg, max_instances=M + M // 4)
+    held = R.rasterizer.workspace_bytes()
+    for rep in range(3):
+        out = R.render(dev, cams, cfg, reserved=True)
+        st = R.status(len(cams))
+        assert [s.total_splat_instances for s in st] == [s.total_splat_instances for s in plain.stats]
+        assert torch.equal(out.rgb, plain.rgb) and torch.equal(out.depth, plain.depth), rep
+    assert R.rasterizer.workspace_bytes() == held  # nothing allocated
+    # backward after a reserved frame equals the plain one's
+    grads = []
+    for reserved in (False, True):
+        planes = dev.planes.detach().clone().requires_grad_(True)
+        sh = dev.sh.detach().clone().requires_grad_(True)
+        rgb, _ = rasterize_views(planes, sh, cams, cfg, R, reserved=reserved)
+        rgb.sum().backward()
+        torch.cuda.synchronize()
+        grads.append((planes.grad, sh.grad))
+    assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+    # a context sized by a small frame cannot hold the big one
+    small = random_scene(2_000, 1.0, 89).to_device()
+    R2 = MultiViewRenderer(0)
+    R2.render(small, cams[:1], cfg)
+    R2.synchronize()
+    big = R2.render(dev, cams, cfg, reserved=True)
+    with pytest.raises(LumenError) as ei:
+        R2.status(len(cams))
+    assert ei.value.status == -101
+    assert float(big.rgb.abs().max()) == 0.0
+    planes = dev.planes.detach().clone().requires_grad_(True)
+    sh = dev.sh.detach().clone().requires_grad_(True)
+    rgb, _ = rasterize_views(planes, sh, cams, cfg, R2, reserved=True)
+    rgb.sum().backward()
+    torch.cuda.synchronize()
+    assert float(planes.grad.abs().max()) == 0.0 and float(sh.grad.abs().max()) == 0.0
+    R.close()
+    R2.close()

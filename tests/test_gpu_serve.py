@@ -76,3 +76,32 @@ def test_budget_admission_serialises_but_completes(scenes):
     done = server.run(reqs, realtime=False)
     assert len(done) == 12
     assert max(peak) == 1  # a tiny budget admits one frame at a time
+
+
+def test_reserved_frames_that_overflow_are_rendered_again(scenes):
+    """A reserved frame whose views produce more instances than the server
+    expects (the estimate is cut to a few keys) is rendered again plainly;
+    every result still equals the request served alone."""
+    import torch
+    from lumen_amd.serve import Request, RenderServer, poisson_requests
+    reqs = poisson_requests(list(scenes), RES, rate=4000.0, count=16, backward_fraction=0.25,
+                            seed=21, targets=make_target)
+    learner = RenderServer(scenes, slots=2, max_views=4)
+    learner.run([Request(r.rid, r.scene, r.camera, r.backward, r.target) for r in reqs],
+                realtime=False)                 # learns every shape of the stream
+    server = RenderServer(scenes, slots=2, max_views=4)
+    # far too few instances: the slots are reserved for ~64k keys, and the
+    # reserved frames overflow
+    server.instances = {k: 1 for k in learner.instances}
+    learner.close()
+    done = server.run(reqs, realtime=False)
+    say(f"[serve] replanned frames: {server.replans}")
+    assert server.replans > 0
+    alone = RenderServer(scenes, slots=1, max_views=1, reserved=False)
+    for r in done:
+        solo = alone.serve_one(Request(r.rid, r.scene, r.camera, r.backward, r.target))
+        assert torch.equal(solo.rgb, r.rgb), r.rid
+        if r.backward:
+            assert torch.equal(solo.grad_planes, r.grad_planes), r.rid
+    server.close()
+    alone.close()
