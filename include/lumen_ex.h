@@ -113,7 +113,10 @@ int lumen_ex_abi_version(void);
  * key headroom.  Host-only: touches no device.
  *
  * lumen_ex_context_reserve: allocates that workspace in the context; frames
- * within the bounds then allocate nothing (device or pinned host).
+ * within the bounds then allocate nothing (device or pinned host).  Its
+ * bounds (n_views * n_gaussians visible splats, max_instances keys) are also
+ * the capacities of the LUMEN_EX_RESERVED frames that follow, even where
+ * larger frames have grown the buffers beyond them.
  *
  * lumen_ex_context_workspace: device bytes the context holds now. */
 LumenStatus lumen_ex_workspace_size(const LumenExCamera* cameras, uint32_t n_views,
@@ -147,9 +150,11 @@ LumenStatus lumen_ex_render(LumenExContext* ctx, const LumenExScene* scene,
                                cameras, config, outputs and stream are unchanged; no
                                host round trip per frame */
 #define LUMEN_EX_RESERVED 16u /* enqueue the whole frame with the visible-splat and key
-                                 capacities the context already holds (reserved with
-                                 lumen_ex_context_reserve, or grown by earlier frames)
-                                 and return at once: no host round trip, any cameras.
+                                 capacities of the context's last
+                                 lumen_ex_context_reserve (with none: all its buffers
+                                 hold, as grown by earlier frames) and return at once:
+                                 no host round trip, any cameras.  The sort's grids
+                                 cover the key capacity, so reserve tightly.
                                  A frame that exceeds them renders background and
                                  lumen_ex_frame_status reports LUMEN_EX_ERROR_CAPACITY;
                                  on a context with no capacity yet the frame renders

@@ -124,7 +124,11 @@ struct LumenExContext {
   uint32_t nviews = 0;
   uint32_t tile_w = 0, tile_h = 0;
   uint64_t v_cap = 0, m_cap = 0;  // capacities the frame's arrays were sized for
-  uint32_t key_bytes = 8;          // width of the last plain frame's sort keys
+  // the bounds of the last lumen_ex_context_reserve: what a reserved frame
+  // sizes its launches by (the buffers may hold more, from larger frames)
+  bool reserved_set = false;
+  uint64_t reserved_v = 0, reserved_m = 0;
+  uint32_t key_bytes = 8;         // width of the last plain frame's sort keys
   bool sizes_known = false;        // V and M were read back (not a graph replay)
   bool kept = false, pixels = false, has_grad = false;
   GraphCache graph;

@@ -1177,15 +1177,22 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   return LUMEN_SUCCESS;
 }
 
-// The reserved frame: the whole frame enqueued with the capacities the
-// buffers hold, the sizes and the sort plan left on the device (as a graph
-// replay does, without capture): the host returns at once.
+// The reserved frame: the whole frame enqueued with the capacities of the
+// last reservation (all the buffers hold, with none), the sizes and the sort
+// plan left on the device (as a graph replay does, without capture): the host
+// returns at once.  The sort's and the ranges' grids cover the key capacity,
+// so a tight reservation keeps a small frame's launches small in a context
+// that larger frames have grown.
 static LumenStatus render_reserved(LumenExContext* c, const LumenExScene* sc,
                                    const LumenExCamera* cams, uint32_t nviews,
                                    const LumenExRenderConfig* cfg, float* out_rgb,
                                    float* out_depth, hipStream_t st) {
   uint64_t v_now = 0, m_now = 0;
   current_capacity(c, v_now, m_now);
+  if (c->reserved_set) {
+    v_now = std::min(v_now, c->reserved_v);
+    m_now = std::min(m_now, c->reserved_m);
+  }
   if (v_now == 0 || m_now == 0)  // nothing sized yet: a plain frame sizes the buffers
     return render_plain(c, sc, cams, nviews, cfg, out_rgb, out_depth, nullptr, st);
   if ((uint64_t)nviews * sc->count >= 0xFFFFFFFFull)
@@ -1478,6 +1485,11 @@ LumenStatus lumen_ex_context_reserve(LumenExContext* c, const LumenExCamera* cam
   LUMEN_HIP(hipSetDevice(c->device));
   if (c->graph.valid) graph_release(c->graph);  // a growing buffer would move under it
   LUMEN_TRY(workspace(c, need_ensure, n_gaussians, ws, config, max_instances, with_backward != 0));
+  // the capacities reserved frames use from now on (workspace() sizes the
+  // buffers for exactly these)
+  c->reserved_set = true;
+  c->reserved_v = (uint64_t)ws.fs.nviews * n_gaussians;
+  c->reserved_m = (config->flags & LUMEN_EX_GRAPH) ? graph_m_cap(max_instances) : max_instances;
   // the host staging of the per-view parameters
   const size_t vbytes = sizeof(ViewParams) * n_views;
   if (c->pinned_views_cap < vbytes) {

@@ -107,7 +107,11 @@ class RenderServer:
         self.free = list(range(slots))
         self.inflight: List[_Frame] = []
         self.mem_in_use = 0
+        # waiting requests in arrival order, and the forward ones per scene;
+        # an admitted request leaves both lazily (admitted >= 0)
         self.queue: collections.deque = collections.deque()
+        self.fwd_queues: Dict[str, collections.deque] = {}
+        self._ws_memo: Dict[tuple, int] = {}  # frame workspaces by shape
         self.completed: List[Request] = []
         self.t0 = 0.0
         self.frames = 0
@@ -130,34 +134,55 @@ class RenderServer:
         """The device workspace of a frame (lumen_ex_workspace_size).  For a
         view of an unseen shape the instances are taken as 8 per Gaussian at
         1080p, scaled with the image area (the frame that measures it is
-        plain, sized exactly)."""
+        plain, sized exactly).  The workspace depends on the cameras only
+        through their image sizes, so it is kept per (scene, sizes, backward,
+        instances)."""
         n = self.scenes[scene].count
         m = self._expected_instances(scene, cams)
         if m is None:
             m = int(sum(n * max(2.0, 8.0 * c.config.width * c.config.height / (1920.0 * 1080.0))
                         for c in cams))
-        return workspace_size(n, list(cams), self.config, max_instances=m, backward=backward,
-                              save_pixels=backward)
+        key = (scene, tuple(sorted((c.config.width, c.config.height) for c in cams)), backward, m)
+        ws = self._ws_memo.get(key)
+        if ws is None:
+            ws = workspace_size(n, list(cams), self.config, max_instances=m, backward=backward,
+                                save_pixels=backward)
+            if len(self._ws_memo) > 65536:
+                self._ws_memo.clear()
+            self._ws_memo[key] = ws
+        return ws
 
     # -- scheduling ---------------------------------------------------------
     def _now(self) -> float:
         return time.perf_counter() - self.t0
 
+    def _enqueue(self, r: Request) -> None:
+        r.admitted = -1.0
+        self.queue.append(r)
+        if not r.backward:
+            self.fwd_queues.setdefault(r.scene, collections.deque()).append(r)
+
     def _pick_frame(self) -> Optional[List[Request]]:
         """The oldest waiting request and, if it is a forward request, the other
         waiting forward requests of its scene (arrival order) while the frame's
         workspace stays within one slot's share of the budget."""
-        if not self.queue:
+        q = self.queue
+        while q and q[0].admitted >= 0:
+            q.popleft()
+        if not q:
             return None
-        head = self.queue[0]
+        head = q[0]
         if head.backward:
             return [head]
+        fq = self.fwd_queues[head.scene]
+        while fq and fq[0].admitted >= 0:
+            fq.popleft()
         cap = self.budget // len(self.slots)
         group = [head]
-        for r in self.queue:
+        for r in fq:
             if len(group) >= self.max_views:
                 break
-            if r is head or r.backward or r.scene != head.scene:
+            if r is head or r.admitted >= 0:
                 continue
             if self.frame_memory(head.scene, [g.camera for g in group] + [r.camera], False) > cap:
                 continue
@@ -173,12 +198,10 @@ class RenderServer:
         mem = self.frame_memory(group[0].scene, [r.camera for r in group], group[0].backward)
         if self.mem_in_use + mem > self.budget and self.inflight:
             return False  # wait for memory; a lone frame is always admitted
-        for r in group:
-            self.queue.remove(r)
         slot = self.free.pop()
         now = self._now()
         for r in group:
-            r.admitted = now
+            r.admitted = now  # leaves the queues
         frame = self._launch(slot, group)
         frame.mem = mem
         self.mem_in_use += mem
@@ -287,7 +310,7 @@ class RenderServer:
             while i < len(pending) and (not realtime or pending[i].arrival <= now):
                 if not realtime:
                     pending[i].arrival = now
-                self.queue.append(pending[i])
+                self._enqueue(pending[i])
                 i += 1
             progressed = self._retire() > 0
             while self._admit():
