@@ -287,6 +287,32 @@ def test_radix_sort_matches_stable_argsort(rasterizer, n):
     np.testing.assert_array_equal(v, order.astype(np.int32))
 
 
+def test_radix_sort_past_2_pow_30_keys(rasterizer):
+    """2^30 + 3 keys (8 digit passes of 64-bit keys with many ties), checked
+    by their properties on the device: sorted, every output pair (key, value)
+    is the input key at index value, the values are a permutation, and equal
+    keys keep their input order."""
+    import torch
+    n = (1 << 30) + 3
+    g = torch.Generator(device="cuda").manual_seed(30)
+    say(f"[sort] n={n}: 64-bit look-back words")
+    keys = (torch.randint(0, 1 << 16, (n,), generator=g, device="cuda") << 40) | \
+        torch.randint(0, 1 << 10, (n,), generator=g, device="cuda")
+    k = keys.clone()
+    v = torch.arange(n, dtype=torch.int32, device="cuda")
+    rasterizer.radix_sort(k, v)
+    torch.cuda.synchronize()
+    assert bool((k[1:] >= k[:-1]).all())
+    ties = k[1:] == k[:-1]
+    assert bool((v[1:][ties] > v[:-1][ties]).all())
+    idx = v.long()
+    assert bool((keys[idx] == k).all())
+    del keys, k
+    seen = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    seen[idx] = 1
+    assert bool(seen.all()) and int(ties.sum()) > n // 2
+
+
 def test_build_ranges_kat(rasterizer, oracle):
     """tests/test_binning.mojo:10-51."""
     import torch
