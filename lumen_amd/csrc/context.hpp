@@ -108,6 +108,7 @@ struct LumenExContext {
   lumen::DevBuf keys[2], vals[2], keys_keep, vals_keep;
   lumen::DevBuf hist, hist_part;
   lumen::DevBuf ranges, t_final, n_eval, minmax;
+  lumen::DevBuf tile_order;  // the compositors' tile order: heaviest key count class first
   lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
   lumen::DevBuf grad_vis;  // backward: per visible splat dL/d(mean2d, inv cov, opacity, rgb, depth)
   lumen::DevBuf sum_seg, sum_aux;  // backward: per slot chunk, its first splat and partial sums

@@ -584,18 +584,82 @@ __device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views
 // loads and the x terms of the power, i00 dx dx and 2 i01 dx (each pixel's
 // power keeps the reference's expression tree).  A pixel stops at
 // T <= 1e-4, a lane when both of its pixels have, the block when all lanes have.
-template <int BATCH, int PIX>
+// Frame tiles ordered by key count, heaviest first (classes floor(log2 n),
+// any order inside a class): the compositor's blocks take tiles in this
+// order, so the longest tiles start first instead of landing in the launch's
+// tail (a longest-processing-time-first schedule).  Each tile's pixels do not
+// depend on the order.  One block; a wave adds to the class counters once
+// per distinct class among its lanes.
+constexpr int kOrderClasses = 32;
+__device__ __forceinline__ uint32_t tile_class(const int2* __restrict__ ranges, bool over,
+                                               uint32_t i) {
+  const int2 r = over ? make_int2(-1, -1) : ranges[i];
+  const uint32_t n = (r.x >= 0 && r.y > r.x) ? (uint32_t)(r.y - r.x) : 0u;
+  return n ? (uint32_t)__clz(n) : 31u;  // 31 - floor(log2 n): the heaviest class is 0
+}
+// Lanes of the wave whose class equals this lane's (inactive lanes excluded).
+__device__ __forceinline__ uint64_t class_peers(bool valid, uint32_t c) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 5; ++b) {
+    const bool on = (c >> b) & 1u;
+    const uint64_t bb = __ballot(on);
+    peers &= on ? bb : ~bb;
+  }
+  return peers;
+}
+__global__ __launch_bounds__(1024) void k_tile_order(const int2* __restrict__ ranges,
+                                                     uint32_t tiles,
+                                                     const SortPlan* __restrict__ plan,
+                                                     uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kOrderClasses];
+  const int t = threadIdx.x;
+  const bool over = plan->overflow != 0;
+  const uint64_t lt = lanemask_lt();
+  if (t < kOrderClasses) cnt[t] = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    const uint32_t i = base + t;
+    const bool valid = i < tiles;
+    const uint32_t c = valid ? tile_class(ranges, over, i) : 0u;
+    const uint64_t peers = class_peers(valid, c);
+    if (valid && (peers & lt) == 0) atomicAdd(&cnt[c], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  if (t == 0) {  // class starts, heaviest class first
+    uint32_t s = 0;
+    for (int c = 0; c < kOrderClasses; ++c) {
+      const uint32_t n = cnt[c];
+      cnt[c] = s;
+      s += n;
+    }
+  }
+  __syncthreads();
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    const uint32_t i = base + t;
+    const bool valid = i < tiles;
+    const uint32_t c = valid ? tile_class(ranges, over, i) : 0u;
+    const uint64_t peers = class_peers(valid, c);
+    const int leader = peers ? __ffsll((unsigned long long)peers) - 1 : 0;
+    uint32_t first = 0;
+    if (valid && (peers & lt) == 0) first = atomicAdd(&cnt[c], (uint32_t)__popcll(peers));
+    first = __shfl(first, leader, 64);
+    if (valid) order[first + (uint32_t)__popcll(peers & lt)] = i;
+  }
+}
+
+template <int BATCH, int PIX, bool QUAD>
 __global__ __launch_bounds__(256) void k_composite(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
     const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
     const ViewSpan* __restrict__ spans, float* __restrict__ rgb, float* __restrict__ depth,
-    float* __restrict__ t_final, int32_t* __restrict__ n_eval) {
+    float* __restrict__ t_final, int32_t* __restrict__ n_eval, const uint32_t* __restrict__ order) {
   static_assert(BATCH <= 256, "list entries are bytes");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint8_t s_mask[BATCH];
   __shared__ uint8_t s_list[4 / PIX][BATCH];  // one list per wave
-  const uint32_t gt = blockIdx.x;
+  const uint32_t gt = order[blockIdx.x];
   const int vw = view_of_tile(views, nviews, gt);
   const ViewParams& v = views[vw];
   const int nthreads = v.tile_w * v.tile_h / PIX;
@@ -606,8 +670,12 @@ __global__ __launch_bounds__(256) void k_composite(
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
-  const int px = tx * v.tile_w + t % v.tile_w;
-  const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
+  // QUAD: wave w covers the 8x8 block (w % nwx, w / nwx) of the tile, one
+  // pixel per lane; else rows (t / tile_w) PIX .. + PIX - 1, column t % tile_w
+  const int nwx = v.tile_w >> 3;
+  const int px = QUAD ? tx * v.tile_w + (w % nwx) * 8 + (lane & 7) : tx * v.tile_w + t % v.tile_w;
+  const int py0 = QUAD ? ty * v.tile_h + (w / nwx) * 8 + (lane >> 3)
+                       : ty * v.tile_h + (t / v.tile_w) * PIX;
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
   float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
@@ -646,8 +714,11 @@ __global__ __launch_bounds__(256) void k_composite(
         s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
         s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
         s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                        tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
+        s_mask[j] = QUAD ? (uint8_t)block_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                               tx * v.tile_w, ty * v.tile_h, 8, 8, nwx, nwaves)
+                         : (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                               tx * v.tile_w, ty * v.tile_h, v.tile_w, rows,
+                                               nwaves);
       }
       __syncthreads();
       // this wave's list of the batch's splats, in order
@@ -858,8 +929,11 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   return LUMEN_SUCCESS;
 }
 
-static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t threads,
-                                    hipStream_t st, LumenExContext* c, uint32_t nviews,
+constexpr bool kQuadWaves = false;
+constexpr uint32_t kQuadBatch = 0;
+
+static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t tile_w,
+                                    uint32_t tile_h, hipStream_t st, LumenExContext* c, uint32_t nviews,
                                     float* rgb, float* depth, float* t_final, int32_t* n_eval) {
   const int2* ranges = c->ranges.as<int2>();
   const uint32_t* v0 = c->vals[0].as<uint32_t>();
@@ -868,20 +942,31 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t thr
   const float4* rec = c->rec.as<float4>();
   const ViewParams* views = c->views.as<ViewParams>();
   const ViewSpan* spans = c->spans.as<ViewSpan>();
-  // two pixels per lane where the tile has at least 128 pixels
-  const bool two = threads % 128 == 0;
+  const uint32_t threads = tile_w * tile_h;
+  const bool quad = kQuadWaves && tile_w % 8 == 0 && tile_h % 8 == 0;
+  uint32_t* order = c->tile_order.as<uint32_t>();
+  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, (uint32_t)tiles, plan,
+                     order);
+  LUMEN_LAUNCHED();
+  // two pixels per lane where the tile has at least 128 pixels (strips)
+  const bool two = !quad && threads % 128 == 0;
+  if (quad && kQuadBatch) batch = kQuadBatch;
   const dim3 grid((uint32_t)tiles), block(two ? threads / 2 : threads);
-#define LUMEN_COMPOSITE(B, P)                                                                  \
-  hipLaunchKernelGGL((k_composite<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec, views, \
-                     (int)nviews, spans, rgb, depth, t_final, n_eval)
-  if (two) {
-    if (batch == 64) LUMEN_COMPOSITE(64, 2);
-    else if (batch == 128) LUMEN_COMPOSITE(128, 2);
-    else LUMEN_COMPOSITE(256, 2);
+#define LUMEN_COMPOSITE(B, P, Q)                                                               \
+  hipLaunchKernelGGL((k_composite<B, P, Q>), grid, block, 0, st, ranges, v0, v1, plan, rec,    \
+                     views, (int)nviews, spans, rgb, depth, t_final, n_eval, order)
+  if (quad) {
+    if (batch == 64) LUMEN_COMPOSITE(64, 1, true);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 1, true);
+    else LUMEN_COMPOSITE(256, 1, true);
+  } else if (two) {
+    if (batch == 64) LUMEN_COMPOSITE(64, 2, false);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 2, false);
+    else LUMEN_COMPOSITE(256, 2, false);
   } else {
-    if (batch == 64) LUMEN_COMPOSITE(64, 1);
-    else if (batch == 128) LUMEN_COMPOSITE(128, 1);
-    else LUMEN_COMPOSITE(256, 1);
+    if (batch == 64) LUMEN_COMPOSITE(64, 1, false);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 1, false);
+    else LUMEN_COMPOSITE(256, 1, false);
   }
 #undef LUMEN_COMPOSITE
   LUMEN_LAUNCHED();
@@ -906,6 +991,7 @@ static LumenStatus frame_buffers(LumenExContext* c, Need&& need, uint64_t N, con
   LUMEN_TRY(need(c->totals, 32));
   LUMEN_TRY(need(c->plan, sizeof(SortPlan)));
   LUMEN_TRY(need(c->ranges, fs.tiles * 8));
+  LUMEN_TRY(need(c->tile_order, fs.tiles * 4));
   if (save_pixels) {
     LUMEN_TRY(need(c->t_final, fs.pixels * 4));
     LUMEN_TRY(need(c->n_eval, fs.pixels * 4));
@@ -1042,7 +1128,7 @@ static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const F
   const uint32_t tpix = cfg->tile_width * cfg->tile_height;
   const uint32_t batch = cfg->batch_depth ? cfg->batch_depth : (tpix % 128 == 0 ? 128u : 256u);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, false, st));
-  LUMEN_TRY(composite_launch(batch, fs.tiles, cfg->tile_width * cfg->tile_height, st, c,
+  LUMEN_TRY(composite_launch(batch, fs.tiles, cfg->tile_width, cfg->tile_height, st, c,
                              fs.nviews, out_rgb, out_depth, tf, ne));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COMPOSITE, true, st));
   return LUMEN_SUCCESS;
@@ -1329,7 +1415,7 @@ static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
   return {&c->packed, &c->part,     &c->totals,    &c->plan,      &c->views,    &c->spans,
           &c->geo,    &c->block_bounds, &c->rec,   &c->vis2orig,  &c->radius,   &c->vmeta,
           &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
-          &c->hist,   &c->hist_part, &c->ranges,   &c->t_final,   &c->n_eval,   &c->minmax,
+          &c->hist,   &c->hist_part, &c->ranges, &c->tile_order,   &c->t_final,   &c->n_eval,   &c->minmax,
           &c->partial, &c->grad_vis, &c->sum_seg, &c->sum_aux, &c->loss_part, &c->dens_cls,
           &c->dens_cnt, &c->dens_off, &c->dens_misc, &c->segs};
 }
