@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     const ViewSpan* __restrict__ spans, const uint4* __restrict__ vmeta,
     const float* __restrict__ t_final, const int32_t* __restrict__ n_eval,
     const float* __restrict__ g_rgb, const float* __restrict__ g_depth,
-    float* __restrict__ partial) {
+    float* __restrict__ partial, const uint32_t* __restrict__ order) {
   static_assert(BATCH <= 256, "list entries are bytes");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint32_t s_dst[BATCH];  // emission-order slot of each staged instance
@@ -85,9 +85,18 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   __shared__ float s_part[4 / PIX][BATCH * kGradVis];  // per-wave sums, one slot per splat
   __shared__ int s_last;
   if (plan->overflow) return;
-  // each XCD walks a contiguous run of tiles: the slots of a splat covering
-  // neighbouring tiles are neighbours, and their partial lines merge in L2
-  const uint32_t gt = xcd_block(blockIdx.x, gridDim.x);
+  // tiles in the forward's order (heaviest key-count class first, see
+  // k_tile_order), each XCD (grid block i runs on XCD i % 8) taking runs of 8
+  // consecutive positions: every XCD starts on the heavy tiles, and a run's
+  // neighbouring tiles (a wave's tiles of one class keep their order) put
+  // the slots of a splat covering them through one L2
+  uint32_t gt;
+  {
+    const uint32_t n = gridDim.x, x = blockIdx.x % 8u, k = blockIdx.x / 8u;
+    uint32_t pos = (k / 8u) * 64u + x * 8u + k % 8u;
+    if (blockIdx.x >= (n / 64u) * 64u) pos = blockIdx.x;  // the last partial group: as is
+    gt = order[pos];
+  }
   const int vw = view_of_tile_b(views, nviews, gt);
   const ViewParams& v = views[vw];
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
@@ -814,10 +823,12 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
   if (batch == 64)
     hipLaunchKernelGGL((k_composite_bwd<64, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
-                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
+                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part,
+                       c->tile_order.as<uint32_t>());
   else
     hipLaunchKernelGGL((k_composite_bwd<128, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
-                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part);
+                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part,
+                       c->tile_order.as<uint32_t>());
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, true, st));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, false, st));

@@ -648,7 +648,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const int2* __restrict__ ra
   }
 }
 
-template <int BATCH, int PIX, bool QUAD>
+template <int BATCH, int PIX>
 __global__ __launch_bounds__(256) void k_composite(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
@@ -670,12 +670,8 @@ __global__ __launch_bounds__(256) void k_composite(
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
-  // QUAD: wave w covers the 8x8 block (w % nwx, w / nwx) of the tile, one
-  // pixel per lane; else rows (t / tile_w) PIX .. + PIX - 1, column t % tile_w
-  const int nwx = v.tile_w >> 3;
-  const int px = QUAD ? tx * v.tile_w + (w % nwx) * 8 + (lane & 7) : tx * v.tile_w + t % v.tile_w;
-  const int py0 = QUAD ? ty * v.tile_h + (w / nwx) * 8 + (lane >> 3)
-                       : ty * v.tile_h + (t / v.tile_w) * PIX;
+  const int px = tx * v.tile_w + t % v.tile_w;
+  const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
   float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
@@ -714,11 +710,8 @@ __global__ __launch_bounds__(256) void k_composite(
         s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
         s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
         s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-        s_mask[j] = QUAD ? (uint8_t)block_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                               tx * v.tile_w, ty * v.tile_h, 8, 8, nwx, nwaves)
-                         : (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                               tx * v.tile_w, ty * v.tile_h, v.tile_w, rows,
-                                               nwaves);
+        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
+                                        tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
       }
       __syncthreads();
       // this wave's list of the batch's splats, in order
@@ -929,9 +922,6 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   return LUMEN_SUCCESS;
 }
 
-constexpr bool kQuadWaves = false;
-constexpr uint32_t kQuadBatch = 0;
-
 static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t tile_w,
                                     uint32_t tile_h, hipStream_t st, LumenExContext* c, uint32_t nviews,
                                     float* rgb, float* depth, float* t_final, int32_t* n_eval) {
@@ -943,30 +933,24 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t til
   const ViewParams* views = c->views.as<ViewParams>();
   const ViewSpan* spans = c->spans.as<ViewSpan>();
   const uint32_t threads = tile_w * tile_h;
-  const bool quad = kQuadWaves && tile_w % 8 == 0 && tile_h % 8 == 0;
   uint32_t* order = c->tile_order.as<uint32_t>();
   hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, (uint32_t)tiles, plan,
                      order);
   LUMEN_LAUNCHED();
-  // two pixels per lane where the tile has at least 128 pixels (strips)
-  const bool two = !quad && threads % 128 == 0;
-  if (quad && kQuadBatch) batch = kQuadBatch;
+  // two pixels per lane where the tile has at least 128 pixels
+  const bool two = threads % 128 == 0;
   const dim3 grid((uint32_t)tiles), block(two ? threads / 2 : threads);
-#define LUMEN_COMPOSITE(B, P, Q)                                                               \
-  hipLaunchKernelGGL((k_composite<B, P, Q>), grid, block, 0, st, ranges, v0, v1, plan, rec,    \
-                     views, (int)nviews, spans, rgb, depth, t_final, n_eval, order)
-  if (quad) {
-    if (batch == 64) LUMEN_COMPOSITE(64, 1, true);
-    else if (batch == 128) LUMEN_COMPOSITE(128, 1, true);
-    else LUMEN_COMPOSITE(256, 1, true);
-  } else if (two) {
-    if (batch == 64) LUMEN_COMPOSITE(64, 2, false);
-    else if (batch == 128) LUMEN_COMPOSITE(128, 2, false);
-    else LUMEN_COMPOSITE(256, 2, false);
+#define LUMEN_COMPOSITE(B, P)                                                                  \
+  hipLaunchKernelGGL((k_composite<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec, views, \
+                     (int)nviews, spans, rgb, depth, t_final, n_eval, order)
+  if (two) {
+    if (batch == 64) LUMEN_COMPOSITE(64, 2);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 2);
+    else LUMEN_COMPOSITE(256, 2);
   } else {
-    if (batch == 64) LUMEN_COMPOSITE(64, 1, false);
-    else if (batch == 128) LUMEN_COMPOSITE(128, 1, false);
-    else LUMEN_COMPOSITE(256, 1, false);
+    if (batch == 64) LUMEN_COMPOSITE(64, 1);
+    else if (batch == 128) LUMEN_COMPOSITE(128, 1);
+    else LUMEN_COMPOSITE(256, 1);
   }
 #undef LUMEN_COMPOSITE
   LUMEN_LAUNCHED();
