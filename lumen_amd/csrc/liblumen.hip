@@ -608,12 +608,21 @@ __device__ __forceinline__ uint64_t class_peers(bool valid, uint32_t c) {
   }
   return peers;
 }
+// Frames of more tiles than this keep their tiles in order (written by a
+// grid of blocks): one block would take longer to order them than the
+// launch's tail costs, which is short against so many tiles.
+constexpr uint32_t kOrderMaxTiles = 65536;
 __global__ __launch_bounds__(1024) void k_tile_order(const int2* __restrict__ ranges,
                                                      uint32_t tiles,
                                                      const SortPlan* __restrict__ plan,
                                                      uint32_t* __restrict__ order) {
   __shared__ uint32_t cnt[kOrderClasses];
   const int t = threadIdx.x;
+  if (tiles > kOrderMaxTiles) {
+    const uint32_t i = blockIdx.x * 1024u + (uint32_t)t;
+    if (i < tiles) order[i] = i;
+    return;
+  }
   const bool over = plan->overflow != 0;
   const uint64_t lt = lanemask_lt();
   if (t < kOrderClasses) cnt[t] = 0;
@@ -934,7 +943,8 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t til
   const ViewSpan* spans = c->spans.as<ViewSpan>();
   const uint32_t threads = tile_w * tile_h;
   uint32_t* order = c->tile_order.as<uint32_t>();
-  hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, st, ranges, (uint32_t)tiles, plan,
+  const uint32_t ob = tiles > kOrderMaxTiles ? div_up((uint32_t)tiles, 1024u) : 1u;
+  hipLaunchKernelGGL(k_tile_order, dim3(ob), dim3(1024), 0, st, ranges, (uint32_t)tiles, plan,
                      order);
   LUMEN_LAUNCHED();
   // two pixels per lane where the tile has at least 128 pixels
