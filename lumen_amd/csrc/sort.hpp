@@ -96,10 +96,25 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
     const uint32_t idx = base + j * kSortThreads + t;
     kk[j] = idx < end ? keys[idx] : K(0);
   }
+  // A splat's keys are consecutive and share their depth digits, so
+  // neighbouring lanes often hold the same digit: each run of equal digits
+  // in neighbouring lanes adds its length once, from its first lane (fewer
+  // LDS atomics, and fewer lanes of one atomic on the same bin).
+  const int lane = t & 63;
+  const uint64_t above = ~((2ull << lane) - 1ull);  // lanes after this one
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
-    if (base + j * kSortThreads + t < end)
-      atomicAdd(&h[w][(uint32_t)(kk[j] >> shift) & mask], 1u);
+    const bool valid = base + j * kSortThreads + t < end;
+    const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
+    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
+    const bool head = valid && (lane == 0 || prev != d);
+    const uint64_t heads = __ballot(head);
+    const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // valid lanes: 0 .. nvalid - 1
+    if (head) {
+      const uint64_t later = heads & above;
+      const uint32_t next = later ? (uint32_t)__ffsll((unsigned long long)later) - 1u : 64u;
+      atomicAdd(&h[w][d], min(next, nvalid) - (uint32_t)lane);
+    }
   }
   __syncthreads();
   for (int d = t; d < kDigitBins; d += kSortThreads)
