@@ -116,7 +116,11 @@ struct LumenExContext {
   // lumen_ex_densify_and_prune: per-Gaussian class, output count and offset;
   // stats, total and the scan's partial sums
   lumen::DevBuf dens_cls, dens_cnt, dens_off, dens_misc;
-  void* pinned = nullptr;       // small host staging (totals, bounds, plan)
+  void* pinned = nullptr;       // small host staging (plan, bounds)
+  // mapped, coherent host words k_plan writes a plain frame's sizes to (the
+  // host pointer and the device's address of them)
+  uint64_t* ret_host = nullptr;
+  uint64_t* ret_dev = nullptr;
   void* pinned_views = nullptr; // host staging of the per-view parameters
   size_t pinned_views_cap = 0;
   // the last frame (for lumen_ex_copy_stage and the backward)

@@ -89,19 +89,32 @@ struct GeoCache {
 
 // Pass 1: visibility and tile count per (view, Gaussian), the geometry of
 // the visible ones, and per-block bounds of depth keys and view-local tile
-// ids (which decide the sort digits; reduced by k_reduce_bounds).  The
-// packed word's count half is scanned modulo 2^32, so each block also sums
-// its counts exactly in 64 bits (block_inst): the frame's exact instance
-// total, against which k_plan and the host reject frames the 32-bit offsets
-// cannot address.
+// ids (which decide the sort digits; k_plan folds them).  The packed word's
+// count half is scanned modulo 2^32, so each block also sums its counts
+// exactly in 64 bits (block_inst): the frame's exact instance total, against
+// which k_plan and the host reject frames the 32-bit offsets cannot address.
+// INL: the views come in the kernel's arguments (iv) instead of device
+// memory, and block 0 writes them to views_out for the frame's later kernels
+// (a frame of at most kInlineViews views needs no copy of its own).
+constexpr int kInlineViews = 4;
+struct InlineViews {
+  ViewParams v[kInlineViews];
+};
+static_assert(sizeof(ViewParams) % 4 == 0 && kInlineViews * sizeof(ViewParams) / 4 <= 256,
+              "block 0 copies the views a word per thread");
+template <bool INL>
 __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
                                                int nviews, uint64_t* __restrict__ packed,
                                                GeoCache* __restrict__ geo,
                                                uint4* __restrict__ block_bounds,
-                                               uint64_t* __restrict__ block_inst) {
+                                               uint64_t* __restrict__ block_inst, InlineViews iv,
+                                               ViewParams* __restrict__ views_out) {
   __shared__ uint32_t sb[4][4];
   __shared__ uint64_t sinst[4];
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (INL && blockIdx.x == 0 && threadIdx.x < nviews * (int)(sizeof(ViewParams) / 4))
+    reinterpret_cast<uint32_t*>(views_out)[threadIdx.x] =
+        reinterpret_cast<const uint32_t*>(iv.v)[threadIdx.x];
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
   uint64_t inst = 0ull;
   if (i < s.n) {
@@ -110,7 +123,7 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
     const Quat q{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
     const float op = s.op[i];
     for (int vw = 0; vw < nviews; ++vw) {
-      const ViewParams& v = views[vw];
+      const ViewParams& v = INL ? iv.v[vw] : views[vw];
       SplatGeom g;
       uint64_t val = 0;
       const size_t idx = (size_t)vw * s.n + i;
@@ -160,71 +173,86 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
   }
 }
 
-// One block folds the per-block bounds of k_count into bounds[0..3] and
-// their exact instance sums into the 64-bit total at bounds + 4.
-__global__ __launch_bounds__(1024) void k_reduce_bounds(const uint4* __restrict__ block_bounds,
-                                                        const uint64_t* __restrict__ block_inst,
-                                                        uint32_t nblocks,
-                                                        uint32_t* __restrict__ bounds) {
-  __shared__ uint32_t sb[16][4];
-  __shared__ uint64_t sinst[16];
-  uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
-  uint64_t inst = 0ull;
-  for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) {
-    const uint4 x = block_bounds[b];
-    dmin = min(dmin, x.x);
-    dmax = max(dmax, x.y);
-    tmin = min(tmin, x.z);
-    tmax = max(tmax, x.w);
-    inst += block_inst[b];
-  }
-  inst = wave_sum(inst);
-  dmin = wave_min_u32_all(dmin);
-  dmax = wave_max_u32_all(dmax);
-  tmin = wave_min_u32_all(tmin);
-  tmax = wave_max_u32_all(tmax);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sb[w][0] = dmin;
-    sb[w][1] = dmax;
-    sb[w][2] = tmin;
-    sb[w][3] = tmax;
-    sinst[w] = inst;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < 16; ++k) {
-      dmin = min(dmin, sb[k][0]);
-      dmax = max(dmax, sb[k][1]);
-      tmin = min(tmin, sb[k][2]);
-      tmax = max(tmax, sb[k][3]);
-      inst += sinst[k];
-    }
-    bounds[0] = dmin;
-    bounds[1] = dmax;
-    bounds[2] = tmin;
-    bounds[3] = tmax;
-    *reinterpret_cast<uint64_t*>(bounds + 4) = inst;
-  }
-}
-
 // Sort plan, clamped sizes and per-view spans from the scan and the bounds.
-// A frame whose exact instance total (bounds + 4) exceeds the key capacity,
-// or the 2^31 - 1 that int32 tile ranges address (which also covers a count
-// half that wrapped in the scan), is flagged as overflowed: k_emit then
-// writes nothing and the frame renders background.
-__global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
-                       const uint32_t* __restrict__ bounds, uint32_t n, int nviews,
-                       uint64_t m_cap, uint64_t v_cap, SortPlan* __restrict__ plan,
-                       ViewSpan* __restrict__ spans, SortSeg* __restrict__ segs,
-                       int2* __restrict__ ranges, uint32_t tiles, int force_items) {
+// Block 0 first folds k_count's per-block key bounds into bounds[0..3] and
+// their exact instance sums into the 64-bit total at bounds + 4.
+// A frame whose exact instance total exceeds the key capacity, or the
+// 2^31 - 1 that int32 tile ranges address (which also covers a count half
+// that wrapped in the scan), is flagged as overflowed: k_emit then writes
+// nothing and the frame renders background.  With host_ret (mapped, coherent
+// host memory) block 0 also writes the scan total, the four bounds and the
+// exact instance total there, the 32 bytes a plain frame sizes itself by.
+__global__ __launch_bounds__(1024) void k_plan(
+    const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
+    const uint4* __restrict__ block_bounds, const uint64_t* __restrict__ block_inst,
+    uint32_t nblocks, uint32_t* __restrict__ bounds, uint32_t n, int nviews, uint64_t m_cap,
+    uint64_t v_cap, SortPlan* __restrict__ plan, ViewSpan* __restrict__ spans,
+    SortSeg* __restrict__ segs, int2* __restrict__ ranges, uint32_t tiles, int force_items,
+    uint64_t* __restrict__ host_ret) {
   // every block: empty tile ranges (-1, -1) before k_ranges fills the used ones
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
     ranges[t] = make_int2(-1, -1);
   if (blockIdx.x != 0) return;
+  __shared__ uint32_t sb[16][4];
+  __shared__ uint64_t sinst[16];
+  __shared__ uint32_t s_bounds[4];
+  __shared__ uint64_t s_exact;
+  {
+    uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
+    uint64_t inst = 0ull;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += 4096u) {
+      uint4 x[4];
+      uint64_t y[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // four loads of each kind in flight
+        const uint32_t b = b0 + (uint32_t)j * 1024u + threadIdx.x;
+        x[j] = b < nblocks ? block_bounds[b] : make_uint4(0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u);
+        y[j] = b < nblocks ? block_inst[b] : 0ull;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        dmin = min(dmin, x[j].x);
+        dmax = max(dmax, x[j].y);
+        tmin = min(tmin, x[j].z);
+        tmax = max(tmax, x[j].w);
+        inst += y[j];
+      }
+    }
+    inst = wave_sum(inst);
+    dmin = wave_min_u32_all(dmin);
+    dmax = wave_max_u32_all(dmax);
+    tmin = wave_min_u32_all(tmin);
+    tmax = wave_max_u32_all(tmax);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      sb[w][0] = dmin;
+      sb[w][1] = dmax;
+      sb[w][2] = tmin;
+      sb[w][3] = tmax;
+      sinst[w] = inst;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (int)(blockDim.x >> 6);
+      for (int k = 1; k < nw; ++k) {
+        dmin = min(dmin, sb[k][0]);
+        dmax = max(dmax, sb[k][1]);
+        tmin = min(tmin, sb[k][2]);
+        tmax = max(tmax, sb[k][3]);
+        inst += sinst[k];
+      }
+      s_bounds[0] = bounds[0] = dmin;
+      s_bounds[1] = bounds[1] = dmax;
+      s_bounds[2] = bounds[2] = tmin;
+      s_bounds[3] = bounds[3] = tmax;
+      s_exact = inst;
+      *reinterpret_cast<uint64_t*>(bounds + 4) = inst;
+    }
+    __syncthreads();
+  }
   const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
-  const uint64_t exact_m = *reinterpret_cast<const uint64_t*>(bounds + 4);
+  const uint64_t exact_m = s_exact;
   const bool over = exact_m > m_cap || exact_m >= 0x7FFFFFFFull || (uint64_t)V > v_cap;
   // keys per thread of the sort blocks: a graph fixes it, a plain frame takes
   // it from its key count (the host decides the same from the read-back)
@@ -232,7 +260,7 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
   const uint32_t tile = 256u * (uint32_t)items;
   if (threadIdx.x == 0) {
     SortPlan p;
-    plan_digits(V > 0, bounds[0], bounds[1], bounds[2], bounds[3], p);
+    plan_digits(V > 0, s_bounds[0], s_bounds[1], s_bounds[2], s_bounds[3], p);
     plan->passes = p.passes;
     for (int k = 0; k < 8; ++k) {
       plan->shift[k] = p.shift[k];
@@ -246,6 +274,12 @@ __global__ void k_plan(const uint64_t* __restrict__ offsets, const uint64_t* __r
     plan->v = (uint64_t)V <= v_cap ? V : (uint32_t)v_cap;
     plan->overflow = over ? 1u : 0u;
     plan->items = items;
+    if (host_ret) {
+      host_ret[0] = tot;
+      host_ret[1] = (uint64_t)s_bounds[0] | ((uint64_t)s_bounds[1] << 32);
+      host_ret[2] = (uint64_t)s_bounds[2] | ((uint64_t)s_bounds[3] << 32);
+      host_ret[3] = exact_m;
+    }
   }
   const uint32_t mc = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
   for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
@@ -588,8 +622,10 @@ __device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views
 // any order inside a class): the compositor's blocks take tiles in this
 // order, so the longest tiles start first instead of landing in the launch's
 // tail (a longest-processing-time-first schedule).  Each tile's pixels do not
-// depend on the order.  One block; a wave adds to the class counters once
-// per distinct class among its lanes.
+// depend on the order.  kOrderBlocks blocks each count every tile's class
+// (per class: all tiles, and the tiles before the block's own range), then
+// place their own range's tiles; a wave adds to a class counter once per
+// distinct class among its lanes.
 constexpr int kOrderClasses = 32;
 __device__ __forceinline__ uint32_t tile_class(const int2* __restrict__ ranges, bool over,
                                                uint32_t i) {
@@ -609,45 +645,69 @@ __device__ __forceinline__ uint64_t class_peers(bool valid, uint32_t c) {
   return peers;
 }
 // Frames of more tiles than this keep their tiles in order (written by a
-// grid of blocks): one block would take longer to order them than the
-// launch's tail costs, which is short against so many tiles.
+// grid of blocks): ordering them would take longer than the launch's tail
+// costs, which is short against so many tiles.
 constexpr uint32_t kOrderMaxTiles = 65536;
+constexpr uint32_t kOrderBlocks = 8;
+// Tiles per ordering block: a multiple of 64, so each wave's tiles lie on one
+// side of every block's range start.
+__host__ __device__ inline uint32_t order_chunk(uint32_t tiles) {
+  return (tiles + kOrderBlocks * 64u - 1u) / (kOrderBlocks * 64u) * 64u;
+}
 __global__ __launch_bounds__(1024) void k_tile_order(const int2* __restrict__ ranges,
                                                      uint32_t tiles,
                                                      const SortPlan* __restrict__ plan,
                                                      uint32_t* __restrict__ order) {
-  __shared__ uint32_t cnt[kOrderClasses];
+  constexpr int kLoads = 8;  // range loads a thread has in flight
+  __shared__ uint32_t cnt[kOrderClasses], pre[kOrderClasses];
+  __shared__ uint8_t s_cls[kOrderMaxTiles / kOrderBlocks];  // the block's own tiles' classes
   const int t = threadIdx.x;
   if (tiles > kOrderMaxTiles) {
     const uint32_t i = blockIdx.x * 1024u + (uint32_t)t;
     if (i < tiles) order[i] = i;
     return;
   }
+  const uint32_t chunk = order_chunk(tiles);
+  const uint32_t lo = blockIdx.x * chunk;
+  const uint32_t hi = min(tiles, lo + chunk);
   const bool over = plan->overflow != 0;
   const uint64_t lt = lanemask_lt();
-  if (t < kOrderClasses) cnt[t] = 0;
+  if (t < kOrderClasses) cnt[t] = pre[t] = 0;
   __syncthreads();
-  for (uint32_t base = 0; base < tiles; base += 1024) {
-    const uint32_t i = base + t;
-    const bool valid = i < tiles;
-    const uint32_t c = valid ? tile_class(ranges, over, i) : 0u;
-    const uint64_t peers = class_peers(valid, c);
-    if (valid && (peers & lt) == 0) atomicAdd(&cnt[c], (uint32_t)__popcll(peers));
+  for (uint32_t base = 0; base < tiles; base += 1024u * kLoads) {
+    uint32_t c[kLoads];
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      const uint32_t i = base + (uint32_t)j * 1024u + (uint32_t)t;
+      c[j] = i < tiles ? tile_class(ranges, over, i) : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kLoads; ++j) {
+      const uint32_t i = base + (uint32_t)j * 1024u + (uint32_t)t;
+      const bool valid = i < tiles;
+      if (i >= lo && i < hi) s_cls[i - lo] = (uint8_t)c[j];
+      const uint64_t peers = class_peers(valid, c[j]);
+      if (valid && (peers & lt) == 0) {
+        const uint32_t k = (uint32_t)__popcll(peers);
+        atomicAdd(&cnt[c[j]], k);
+        if (i < lo) atomicAdd(&pre[c[j]], k);  // the whole wave lies before lo
+      }
+    }
   }
   __syncthreads();
-  if (t == 0) {  // class starts, heaviest class first
+  if (t == 0) {  // the block's first position in each class, heaviest class first
     uint32_t s = 0;
     for (int c = 0; c < kOrderClasses; ++c) {
       const uint32_t n = cnt[c];
-      cnt[c] = s;
+      cnt[c] = s + pre[c];
       s += n;
     }
   }
   __syncthreads();
-  for (uint32_t base = 0; base < tiles; base += 1024) {
+  for (uint32_t base = lo; base < hi; base += 1024) {
     const uint32_t i = base + t;
-    const bool valid = i < tiles;
-    const uint32_t c = valid ? tile_class(ranges, over, i) : 0u;
+    const bool valid = i < hi;
+    const uint32_t c = valid ? (uint32_t)s_cls[i - lo] : 0u;
     const uint64_t peers = class_peers(valid, c);
     const int leader = peers ? __ffsll((unsigned long long)peers) - 1 : 0;
     uint32_t first = 0;
@@ -886,12 +946,15 @@ struct FrameShape {
   uint32_t nviews;
   uint64_t tiles;   // frame tiles (sum over views)
   uint64_t pixels;  // frame pixels (sum over views)
+  bool inline_views = false;  // k_count takes the views as arguments and writes the device copy
 };
 
-// Per-view parameters on the host and their device copy (through pinned
-// staging, ordered on the stream).
+// Per-view parameters on the host and their device copy: through pinned
+// staging, ordered on the stream, or (allow_inline, at most kInlineViews
+// views, a frame that runs k_count) written by k_count from its arguments.
 static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, uint32_t n,
-                                const LumenExRenderConfig* cfg, FrameShape& fs, hipStream_t st) {
+                                const LumenExRenderConfig* cfg, FrameShape& fs, hipStream_t st,
+                                bool allow_inline = false) {
   c->host_views.resize(n);
   uint64_t tb = 0, pb = 0;
   for (uint32_t k = 0; k < n; ++k) {
@@ -908,6 +971,13 @@ static LumenStatus upload_views(LumenExContext* c, const LumenExCamera* cams, ui
   fs.tiles = tb;
   fs.pixels = pb;
   const size_t bytes = sizeof(ViewParams) * n;
+  if (allow_inline && n <= (uint32_t)kInlineViews) {
+    LUMEN_TRY(ensure(c->views, bytes));
+    LUMEN_TRY(ensure(c->spans, sizeof(ViewSpan) * n));
+    LUMEN_TRY(ensure(c->segs, sizeof(SortSeg) * n));
+    fs.inline_views = true;
+    return LUMEN_SUCCESS;
+  }
   // the staging buffer is reused: the previous frame's copy out of it (which
   // may still be queued behind that frame's work) must have run
   if (c->ev_views_pending) {
@@ -943,7 +1013,7 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t til
   const ViewSpan* spans = c->spans.as<ViewSpan>();
   const uint32_t threads = tile_w * tile_h;
   uint32_t* order = c->tile_order.as<uint32_t>();
-  const uint32_t ob = tiles > kOrderMaxTiles ? div_up((uint32_t)tiles, 1024u) : 1u;
+  const uint32_t ob = tiles > kOrderMaxTiles ? div_up((uint32_t)tiles, 1024u) : kOrderBlocks;
   hipLaunchKernelGGL(k_tile_order, dim3(ob), dim3(1024), 0, st, ranges, (uint32_t)tiles, plan,
                      order);
   LUMEN_LAUNCHED();
@@ -1040,11 +1110,18 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
   const uint32_t nblocks = div_up(sd.n, 256);
   uint4* bb = c->block_bounds.as<uint4>();
   uint64_t* binst = reinterpret_cast<uint64_t*>(bb + nblocks);
-  hipLaunchKernelGGL(k_count, dim3(nblocks), dim3(256), 0, st, sd, c->views.as<ViewParams>(),
-                     (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(), bb, binst);
-  LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_reduce_bounds, dim3(1), dim3(1024), 0, st, bb, binst, nblocks,
-                     frame_bounds(c));
+  InlineViews iv;
+  memset(&iv, 0, sizeof(iv));
+  if (fs.inline_views) {
+    memcpy(iv.v, c->host_views.data(), sizeof(ViewParams) * fs.nviews);
+    hipLaunchKernelGGL(k_count<true>, dim3(nblocks), dim3(256), 0, st, sd,
+                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
+                       c->geo.as<GeoCache>(), bb, binst, iv, c->views.as<ViewParams>());
+  } else {
+    hipLaunchKernelGGL(k_count<false>, dim3(nblocks), dim3(256), 0, st, sd,
+                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
+                       c->geo.as<GeoCache>(), bb, binst, iv, nullptr);
+  }
   LUMEN_LAUNCHED();
   const uint64_t nv = (uint64_t)fs.nviews * sd.n;
   LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
@@ -1056,17 +1133,27 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
 // Stage 2 of a frame: plan and emission, with the given capacities (exact
 // after a host read-back, reserved in a graph, or the buffers' present size
 // ahead of the read-back; k_plan and k_emit flag a frame that exceeds them).
+// With read_back, k_plan also writes the frame's sizes to the context's
+// mapped host words and c->ev_totals is recorded right after it.
 static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
                                 uint64_t v_cap, uint64_t m_cap, int force_items,
-                                hipStream_t st) {
+                                hipStream_t st, bool read_back = false) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
-    const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 256 * 16);
-    hipLaunchKernelGGL(k_plan, dim3(pb), dim3(256), 0, st, c->packed.as<uint64_t>(),
-                       c->totals.as<uint64_t>(), frame_bounds(c), sd.n, (int)fs.nviews, m_cap,
-                       v_cap, plan, c->spans.as<ViewSpan>(), c->segs.as<SortSeg>(),
-                       c->ranges.as<int2>(), (uint32_t)fs.tiles, force_items);
+    const uint32_t nblocks = div_up(sd.n, 256);
+    const uint4* bb = c->block_bounds.as<uint4>();
+    const uint64_t* binst = reinterpret_cast<const uint64_t*>(bb + nblocks);
+    const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 1024 * 16);
+    hipLaunchKernelGGL(k_plan, dim3(pb), dim3(1024), 0, st, c->packed.as<uint64_t>(),
+                       c->totals.as<uint64_t>(), bb, binst, nblocks, frame_bounds(c), sd.n,
+                       (int)fs.nviews, m_cap, v_cap, plan, c->spans.as<ViewSpan>(),
+                       c->segs.as<SortSeg>(), c->ranges.as<int2>(), (uint32_t)fs.tiles,
+                       force_items, read_back ? c->ret_dev : nullptr);
     LUMEN_LAUNCHED();
+    if (read_back) {
+      if (!c->ev_totals) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_totals, hipEventDisableTiming));
+      LUMEN_HIP(hipEventRecord(c->ev_totals, st));
+    }
   } else {
     LUMEN_TRY(fill<uint32_t>(c->spans.as<uint32_t>(), 0u, 4ull * fs.nviews, st));
     LUMEN_TRY(fill<int32_t>(c->ranges.as<int32_t>(), -1, 2ull * fs.tiles, st));
@@ -1187,7 +1274,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   c->pixels = false;
   c->has_grad = false;
   FrameShape fs;
-  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st, /*allow_inline=*/sc->count > 0));
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
   LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
   const SceneDev sd = scene_dev(sc);
@@ -1198,25 +1285,24 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   uint64_t v_now = 0, m_now = 0;
   bool ahead = false;  // plan and emission enqueued before the read-back
   if (sd.n > 0) {
-    uint8_t* h = static_cast<uint8_t*>(c->pinned);
-    // totals, bounds, exact instance total
-    LUMEN_HIP(hipMemcpyAsync(h, c->totals.ptr, 32, hipMemcpyDeviceToHost, st));
-    if (!c->ev_totals) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_totals, hipEventDisableTiming));
-    LUMEN_HIP(hipEventRecord(c->ev_totals, st));
     // Plan and emission go ahead with the capacity the buffers already hold,
-    // so the device works on them while the host waits for the totals (an
-    // event on the copy, not the stream); a frame that outgrows them is
-    // flagged by k_plan / k_emit and both run again below.
+    // so the device works on them while the host waits for the sizes k_plan
+    // writes to mapped host memory (an event after k_plan, not the stream);
+    // a frame that outgrows them is flagged by k_plan / k_emit and both run
+    // again below.  Without capacity only k_plan runs ahead.
     current_capacity(c, v_now, m_now);
-    if (v_now > 0 && m_now > 0) {
-      LUMEN_TRY(enqueue_emit(c, sd, fs, v_now, m_now, 0, st));
-      ahead = true;
-    }
+    ahead = v_now > 0 && m_now > 0;
+    if (!ahead) v_now = m_now = 0;
+    LUMEN_TRY(enqueue_emit(c, sd, fs, v_now, m_now, 0, st, /*read_back=*/true));
     LUMEN_HIP(hipEventSynchronize(c->ev_totals));
-    uint64_t tot;
-    memcpy(&tot, h, 8);
-    memcpy(hb, h + 8, 16);
-    memcpy(&M, h + 24, 8);  // exact: the scan's count half is modulo 2^32
+    const volatile uint64_t* h = c->ret_host;
+    const uint64_t tot = h[0];
+    const uint64_t b01 = h[1], b23 = h[2];
+    hb[0] = (uint32_t)b01;
+    hb[1] = (uint32_t)(b01 >> 32);
+    hb[2] = (uint32_t)b23;
+    hb[3] = (uint32_t)(b23 >> 32);
+    M = h[3];  // exact: the scan's count half is modulo 2^32
     V = tot & 0xFFFFFFFFull;
     if (M >= 0x7FFFFFFFull) {
       if (ahead) LUMEN_HIP(hipStreamSynchronize(st));
@@ -1283,7 +1369,7 @@ static LumenStatus render_reserved(LumenExContext* c, const LumenExScene* sc,
   c->pixels = false;
   c->has_grad = false;
   FrameShape fs;
-  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st, /*allow_inline=*/sc->count > 0));
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
   LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
   const int items = sort_items_for(m_now);
@@ -1507,6 +1593,16 @@ LumenExContext* lumen_ex_context_create(int device) {
     delete c;
     return nullptr;
   }
+  void* ret = nullptr;
+  if (hipHostMalloc(&ret, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&c->ret_dev), ret, 0) != hipSuccess) {
+    if (ret) (void)hipHostFree(ret);
+    (void)hipHostFree(c->pinned);
+    set_error("lumen_ex_context_create: mapped host memory unavailable");
+    delete c;
+    return nullptr;
+  }
+  c->ret_host = static_cast<uint64_t*>(ret);
   return c;
 }
 
@@ -1521,6 +1617,7 @@ void lumen_ex_context_destroy(LumenExContext* c) {
   if (c->ev_views) (void)hipEventDestroy(c->ev_views);
   for (DevBuf* b : context_buffers(c)) release(*b);
   if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ret_host) (void)hipHostFree(c->ret_host);
   if (c->pinned_views) (void)hipHostFree(c->pinned_views);
   delete c;
 }

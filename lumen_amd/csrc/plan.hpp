@@ -74,6 +74,16 @@ __host__ __device__ inline int varying_digits(uint64_t lo, uint64_t hi, int digi
   return n;
 }
 
+// How many digits varying_digits appends for [lo, hi].
+__host__ __device__ inline int varying_count(uint64_t lo, uint64_t hi, int max_digits) {
+  const uint64_t x = lo ^ hi;
+  if (x == 0) return 0;
+  int top = 63;
+  while (!((x >> top) & 1ull)) --top;
+  const int n = top / 8 + 1;
+  return n < max_digits ? n : max_digits;
+}
+
 __host__ __device__ inline int bit_width(uint32_t x) {
   int n = 0;
   while (x) {
@@ -90,6 +100,9 @@ __host__ __device__ inline int bit_width(uint32_t x) {
 // passes, shift, width, mode, tmin, dmin, dbits.
 __host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dmax, uint32_t tmin,
                                             uint32_t tmax, SortPlan& p) {
+  // (every array index below is a constant once the loops unroll, so a
+  // device copy keeps the plan in registers)
+#pragma unroll
   for (int k = 0; k < 8; ++k) p.shift[k] = p.width[k] = 0;
   p.passes = 0;
   p.mode = 1;
@@ -104,20 +117,26 @@ __host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dm
     const int w = np ? (total + np - 1) / np : 0;
     p.dbits = db;
     p.passes = np;
-    for (int k = 0; k < np; ++k) {
-      p.shift[k] = k * w;
-      p.width[k] = total - k * w < w ? total - k * w : w;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < np) {
+        p.shift[k] = k * w;
+        p.width[k] = total - k * w < w ? total - k * w : w;
+      }
     }
     return;
   }
+  // the varying 8-bit digits of the depth half (digits 0-3), then of the
+  // tile half (digits 4-7), as varying_digits lists them
   p.mode = 0;
-  int32_t dg[8];
-  int np = varying_digits(dmin, dmax, 0, 4, dg);
-  np += varying_digits(tmin, tmax, 4, 4, dg + np);
-  p.passes = np;
-  for (int k = 0; k < np; ++k) {
-    p.shift[k] = 8 * dg[k];
-    p.width[k] = 8;
+  const int nd = varying_count(dmin, dmax, 4), nt = varying_count(tmin, tmax, 4);
+  p.passes = nd + nt;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k < nd + nt) {
+      p.shift[k] = 8 * (k < nd ? k : 4 + (k - nd));
+      p.width[k] = 8;
+    }
   }
 }
 
