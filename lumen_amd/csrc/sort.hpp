@@ -13,7 +13,9 @@
 //                   per-wave sub-histograms), stored segment-major then
 //                   digit-major (SortSeg) so that one exclusive scan gives
 //                   every block's start offset per digit
-//   exclusive_scan  over the kDigitBins x blocks table
+//   k_sort_colscan  one block per (segment, digit): the exclusive prefix of
+//                   the digit's column over the segment's blocks, in place,
+//                   and the column's total
 //   k_sort_scatter  stable rank inside the block (256 threads of 12 or 16
 //                   keys, sort_items_for): each wave owns a quarter of the
 //                   block's consecutive keys and ranks them 64 at a time with one
@@ -22,7 +24,11 @@
 //                   one block scan then orders the waves, the block's keys are
 //                   staged in LDS in digit order and written out so that each
 //                   digit's run lands on consecutive addresses (coalesced).
-//                   Equal digits keep their input order.
+//                   Equal digits keep their input order.  A digit's output
+//                   start is the segment's first key, plus the totals of the
+//                   segment's smaller digits (one block scan over the column
+//                   totals), plus the column prefix: the offset an exclusive
+//                   scan of the whole segment-major table would give.
 #pragma once
 
 #include "common.hpp"
@@ -38,7 +44,7 @@ constexpr int kSortThreads = 256;
 // without keys have no blocks (the search takes the last segment starting at
 // or before the block).
 __device__ __forceinline__ bool sort_block(const SortSeg* __restrict__ segs, int nseg, uint32_t i,
-                                           SortSeg& sg, uint32_t& lb) {
+                                           SortSeg& sg, uint32_t& lb, int* seg_out = nullptr) {
   const SortSeg last = segs[nseg - 1];
   const uint32_t n = last.block_base + last.nblocks;
   if (i >= n) return false;
@@ -51,6 +57,7 @@ __device__ __forceinline__ bool sort_block(const SortSeg* __restrict__ segs, int
   }
   sg = segs[lo];
   lb = b - sg.block_base;
+  if (seg_out) *seg_out = lo;
   return true;
 }
 
@@ -121,10 +128,61 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
     hist[hist_index(sg, (uint32_t)d, lb)] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
+// Exclusive prefix sums of each (segment, digit) column of the histogram
+// table over the segment's blocks, in place, and the column totals
+// (totals[segment * kDigitBins + digit]).  Grid: segments x kDigitBins.
+constexpr int kColItems = 8;
+__global__ __launch_bounds__(kSortThreads) void k_sort_colscan(const SortPlan* __restrict__ plan,
+                                                              const SortSeg* __restrict__ segs,
+                                                              int pass, uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ totals) {
+  constexpr int kChunk = kSortThreads * kColItems;
+  __shared__ uint32_t s[kChunk + kChunk / 32];
+  __shared__ uint32_t s_w[kSortThreads / 64];
+  if (pass >= plan->passes) return;
+  const uint32_t seg = blockIdx.x / kDigitBins, d = blockIdx.x % kDigitBins;
+  const SortSeg sg = segs[seg];
+  uint32_t* __restrict__ col = hist + (size_t)sg.block_base * kDigitBins + (size_t)d * sg.nblocks;
+  const int t = threadIdx.x;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < sg.nblocks; b0 += kChunk) {
+    const uint32_t n = min((uint32_t)kChunk, sg.nblocks - b0);
+#pragma unroll
+    for (int j = 0; j < kColItems; ++j) {
+      const int li = j * kSortThreads + t;
+      s[scan_pad(li)] = (uint32_t)li < n ? col[b0 + li] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[kColItems], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kColItems; ++j) {
+      v[j] = s[scan_pad(t * kColItems + j)];
+      sum += v[j];
+    }
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan<uint32_t, kSortThreads>(sum, s_w, tot) + carry;
+#pragma unroll
+    for (int j = 0; j < kColItems; ++j) {
+      s[scan_pad(t * kColItems + j)] = ex;
+      ex += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kColItems; ++j) {
+      const int li = j * kSortThreads + t;
+      if ((uint32_t)li < n) col[b0 + li] = s[scan_pad(li)];
+    }
+    carry += tot;
+    __syncthreads();  // s is refilled next round
+  }
+  if (t == 0) totals[blockIdx.x] = carry;
+}
+
 template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     K* k0, K* k1, uint32_t* v0, uint32_t* v1, const SortPlan* __restrict__ plan,
-    const SortSeg* __restrict__ segs, int nseg, int pass, const uint32_t* __restrict__ offsets) {
+    const SortSeg* __restrict__ segs, int nseg, int pass, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ col_totals) {
   constexpr int kTile = kSortThreads * ITEMS;
   __shared__ K s_key[kTile];
   __shared__ uint32_t s_val[kTile];
@@ -134,11 +192,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   // output position of the block's key at LDS slot i of digit d: s_delta[d] + i
   // (the digit's global start minus its block-local start, modulo 2^32)
   __shared__ uint32_t s_delta[kDigitBins];
-  __shared__ uint32_t s_w[4];
+  __shared__ uint64_t s_w[4];
   if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
   SortSeg sg;
   uint32_t lb;
-  if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
+  int seg_index = 0;
+  if (!sort_block(segs, nseg, blockIdx.x, sg, lb, &seg_index)) return;  // surplus block
   const uint32_t m = sg.key_base + sg.count;  // end of this block's segment
   const K* __restrict__ kin = (pass & 1) ? k1 : k0;
   const uint32_t* __restrict__ vin = (pass & 1) ? v1 : v0;
@@ -168,12 +227,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     kk[j] = idx < m ? kin[idx] : K(0);
     vv[j] = idx < m ? vin[idx] : 0u;
   }
-  // the block's output offsets (digits 2t and 2t + 1), needed only at the
-  // end: loaded after the keys so that their latency hides behind the ranking
+  // the block's column prefixes and the segment's column totals (digits 2t
+  // and 2t + 1), needed only at the end: loaded after the keys so that their
+  // latency hides behind the ranking
   constexpr int kPer = kDigitBins / kSortThreads;
-  uint32_t glob[kPer];
+  uint32_t glob[kPer], ctot[kPer];
 #pragma unroll
-  for (int e = 0; e < kPer; ++e) glob[e] = offsets[hist_index(sg, (uint32_t)(kPer * t + e), lb)];
+  for (int e = 0; e < kPer; ++e) {
+    glob[e] = offsets[hist_index(sg, (uint32_t)(kPer * t + e), lb)];
+    ctot[e] = col_totals[(size_t)seg_index * kDigitBins + kPer * t + e];
+  }
 #pragma unroll
   for (int j = 0; j < ITEMS; ++j) {
     const bool valid = wbase + (uint32_t)j * 64u + (uint32_t)lane < m;
@@ -196,21 +259,29 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
-  {  // thread t: digits 2t and 2t + 1, their block-local starts and each wave's start
-    uint32_t c[kPer][4], sum = 0;
+  {  // thread t: digits 2t and 2t + 1, their block-local starts and each wave's
+     // start, and their starts in the segment (one scan of both: the local
+     // sums stay below 2^32, so the high halves scan the column totals)
+    uint32_t c[kPer][4], sum = 0, tsum = 0;
 #pragma unroll
-    for (int e = 0; e < kPer; ++e)
+    for (int e = 0; e < kPer; ++e) {
+      tsum += ctot[e];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         c[e][i] = s_cnt[i][kPer * t + e];
         sum += c[e][i];
       }
-    uint32_t all;
-    uint32_t st = block_exclusive_scan<uint32_t, kSortThreads>(sum, s_w, all);
+    }
+    uint64_t all;
+    const uint64_t both =
+        block_exclusive_scan<uint64_t, kSortThreads>(((uint64_t)tsum << 32) | sum, s_w, all);
+    uint32_t st = (uint32_t)both;
+    uint32_t dstart = sg.key_base + (uint32_t)(both >> 32);
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
       const int d = kPer * t + e;
-      s_delta[d] = glob[e] - st;
+      s_delta[d] = dstart + glob[e] - st;
+      dstart += ctot[e];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         s_cnt[i][d] = (uint16_t)st;
@@ -247,15 +318,18 @@ struct SortScratch {
 };
 
 // Sort blocks (the grid) for up to m_cap keys in nseg segments, the
-// histogram table (one row per digit per block) and the scan's partial sums.
+// histogram table (one row per digit per block) and the column totals.
 inline uint32_t sort_grid(uint64_t m_cap, int nseg, int items) {
   return div_up(m_cap, (uint64_t)kSortThreads * items) + (uint32_t)nseg;
 }
 inline uint64_t sort_hist_len(uint64_t m_cap, int nseg, int items) {
   return (uint64_t)sort_grid(m_cap, nseg, items) * kDigitBins;
 }
+// The column totals: one per segment and digit.
 inline size_t sort_part_bytes(uint64_t m_cap, int nseg, int items) {
-  return (size_t)div_up(sort_hist_len(m_cap, nseg, items), kScanChunk) * sizeof(uint32_t);
+  (void)m_cap;
+  (void)items;
+  return (size_t)(nseg > 0 ? nseg : 1) * kDigitBins * sizeof(uint32_t);
 }
 
 // The passes of one tile size: `slots` pass slots over (k[0], v[0]).
@@ -264,7 +338,6 @@ inline LumenStatus sort_passes(uint64_t* k[2], uint32_t* v[2], const SortPlan* p
                                const SortSeg* segs, int nseg, uint64_t m_cap, int slots, int modes,
                                uint32_t* hist, uint32_t* part, hipStream_t st) {
   const uint32_t nb = sort_grid(m_cap, nseg, ITEMS);
-  const uint64_t hlen = sort_hist_len(m_cap, nseg, ITEMS);
   uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};
   for (int p = 0; p < slots; ++p) {
     if (modes & 1) {
@@ -277,18 +350,19 @@ inline LumenStatus sort_passes(uint64_t* k[2], uint32_t* v[2], const SortPlan* p
                          k32[0], k32[1], plan, segs, nseg, p, hist);
       LUMEN_LAUNCHED();
     }
-    // the pass's scan runs only when the device plan has the pass, and over
-    // the table rows of the frame's sort blocks (a graph's grid covers its capacity)
-    LUMEN_TRY(exclusive_scan<uint32_t>(hist, hist, hlen, part, nullptr, st,
-                                       ScanGate{&plan->passes, p, &plan->blocks, kDigitBins}));
+    // the pass's column scans run only when the device plan has the pass,
+    // over the columns of the frame's segments (a graph's grid covers its capacity)
+    hipLaunchKernelGGL(k_sort_colscan, dim3((uint32_t)nseg * kDigitBins), dim3(kSortThreads), 0,
+                       st, plan, segs, p, hist, part);
+    LUMEN_LAUNCHED();
     if (modes & 1) {
       hipLaunchKernelGGL((k_sort_scatter<uint64_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st,
-                         k[0], k[1], v[0], v[1], plan, segs, nseg, p, hist);
+                         k[0], k[1], v[0], v[1], plan, segs, nseg, p, hist, part);
       LUMEN_LAUNCHED();
     }
     if (modes & 2) {
       hipLaunchKernelGGL((k_sort_scatter<uint32_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st,
-                         k32[0], k32[1], v[0], v[1], plan, segs, nseg, p, hist);
+                         k32[0], k32[1], v[0], v[1], plan, segs, nseg, p, hist, part);
       LUMEN_LAUNCHED();
     }
   }
