@@ -197,6 +197,12 @@ __global__ __launch_bounds__(1024) void k_plan(
   __shared__ uint64_t sinst[16];
   __shared__ uint32_t s_bounds[4];
   __shared__ uint64_t s_exact;
+  // the scan's total and this thread's first view offsets, loaded with the
+  // fold's loads
+  const uint64_t tot = *totals;
+  const int vw0 = (int)threadIdx.x;
+  const uint64_t off_a0 = vw0 < nviews ? offsets[(size_t)vw0 * n] : 0ull;
+  const uint64_t off_b0 = vw0 + 1 < nviews ? offsets[(size_t)(vw0 + 1) * n] : 0ull;
   {
     uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
     uint64_t inst = 0ull;
@@ -250,7 +256,6 @@ __global__ __launch_bounds__(1024) void k_plan(
     }
     __syncthreads();
   }
-  const uint64_t tot = *totals;
   const uint32_t V = (uint32_t)tot, M = (uint32_t)(tot >> 32);
   const uint64_t exact_m = s_exact;
   const bool over = exact_m > m_cap || exact_m >= 0x7FFFFFFFull || (uint64_t)V > v_cap;
@@ -283,8 +288,8 @@ __global__ __launch_bounds__(1024) void k_plan(
   }
   const uint32_t mc = (uint64_t)M <= m_cap ? M : (uint32_t)m_cap;
   for (int vw = threadIdx.x; vw < nviews; vw += blockDim.x) {
-    const uint64_t a = offsets[(size_t)vw * n];
-    const uint64_t b = vw + 1 < nviews ? offsets[(size_t)(vw + 1) * n] : tot;
+    const uint64_t a = vw == vw0 ? off_a0 : offsets[(size_t)vw * n];
+    const uint64_t b = vw + 1 < nviews ? (vw == vw0 ? off_b0 : offsets[(size_t)(vw + 1) * n]) : tot;
     ViewSpan sp;
     sp.vis_base = (uint32_t)a;
     sp.visible = (uint32_t)b - (uint32_t)a;
@@ -525,8 +530,12 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
                                                 const ViewParams* __restrict__ views,
                                                 int2* __restrict__ ranges, uint32_t tiles) {
   __shared__ uint64_t s_key[kRangeKeys + 1];  // s_key[1 + j]: the block's key j
-  const uint32_t m = plan->m;
   const uint32_t b0 = blockIdx.x * kRangeKeys;
+  // the segment of the block's first key (a search on uniform values that
+  // reads only segs, issued with the plan's loads), then forward with the
+  // thread's keys
+  int sv = seg_of(segs, nseg, b0);
+  const uint32_t m = plan->m;
   if (b0 >= m || plan->overflow) return;  // uniform across the block
   const void* keys = (plan->passes & 1) ? (const void*)k1 : (const void*)k0;
   const SortPlan p = *plan;
@@ -542,9 +551,6 @@ __global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ k0,
 #pragma unroll
   for (int j = 0; j < kRangeItems; ++j) s_key[1 + j * 256 + t] = k[j];
   __syncthreads();
-  // the segment of the block's first key (a search on uniform values), then
-  // forward with the thread's keys
-  int sv = seg_of(segs, nseg, b0);
 #pragma unroll
   for (int j = 0; j < kRangeItems; ++j) {
     const uint32_t i = b0 + (uint32_t)j * 256u + (uint32_t)t;
@@ -629,8 +635,8 @@ __device__ __forceinline__ int view_of_tile(const ViewParams* __restrict__ views
 constexpr int kOrderClasses = 32;
 __device__ __forceinline__ uint32_t tile_class(const int2* __restrict__ ranges, bool over,
                                                uint32_t i) {
-  const int2 r = over ? make_int2(-1, -1) : ranges[i];
-  const uint32_t n = (r.x >= 0 && r.y > r.x) ? (uint32_t)(r.y - r.x) : 0u;
+  const int2 r = ranges[i];  // loaded alongside the flag, then dropped on overflow
+  const uint32_t n = (!over && r.x >= 0 && r.y > r.x) ? (uint32_t)(r.y - r.x) : 0u;
   return n ? (uint32_t)__clz(n) : 31u;  // 31 - floor(log2 n): the heaviest class is 0
 }
 // Lanes of the wave whose class equals this lane's (inactive lanes excluded).
@@ -742,7 +748,8 @@ __global__ __launch_bounds__(256) void k_composite(
   const int px = tx * v.tile_w + t % v.tile_w;
   const int py0 = ty * v.tile_h + (t / v.tile_w) * PIX;
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
-  const int2 rg = plan->overflow ? make_int2(-1, -1) : ranges[gt];
+  const int2 rg0 = ranges[gt];  // loaded alongside the flag
+  const int2 rg = plan->overflow ? make_int2(-1, -1) : rg0;
   float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
   int32_t ne[PIX];  // position of the terminating splat + 1, or the list length
   bool live[PIX];

@@ -82,10 +82,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
                                                            uint32_t* __restrict__ hist) {
   constexpr int kTile = kSortThreads * ITEMS;
   __shared__ uint32_t h[4][kDigitBins];
-  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
+  // the segment lookup reads only segs: it is issued with the plan's loads
   SortSeg sg;
   uint32_t lb;
-  if (!sort_block(segs, nseg, blockIdx.x, sg, lb)) return;  // surplus block
+  const bool mine = sort_block(segs, nseg, blockIdx.x, sg, lb);
+  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
+  if (!mine) return;  // surplus block
   const int t = threadIdx.x;
   const K* __restrict__ keys = (pass & 1) ? k1 : k0;
   const int shift = plan->shift[pass];
@@ -139,9 +141,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_colscan(const SortPlan* _
   constexpr int kChunk = kSortThreads * kColItems;
   __shared__ uint32_t s[kChunk + kChunk / 32];
   __shared__ uint32_t s_w[kSortThreads / 64];
-  if (pass >= plan->passes) return;
   const uint32_t seg = blockIdx.x / kDigitBins, d = blockIdx.x % kDigitBins;
   const SortSeg sg = segs[seg];
+  if (pass >= plan->passes) return;
   uint32_t* __restrict__ col = hist + (size_t)sg.block_base * kDigitBins + (size_t)d * sg.nblocks;
   const int t = threadIdx.x;
   uint32_t carry = 0;
@@ -193,11 +195,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   // (the digit's global start minus its block-local start, modulo 2^32)
   __shared__ uint32_t s_delta[kDigitBins];
   __shared__ uint64_t s_w[4];
-  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
   SortSeg sg;
   uint32_t lb;
   int seg_index = 0;
-  if (!sort_block(segs, nseg, blockIdx.x, sg, lb, &seg_index)) return;  // surplus block
+  const bool mine = sort_block(segs, nseg, blockIdx.x, sg, lb, &seg_index);
+  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
+  if (!mine) return;  // surplus block
   const uint32_t m = sg.key_base + sg.count;  // end of this block's segment
   const K* __restrict__ kin = (pass & 1) ? k1 : k0;
   const uint32_t* __restrict__ vin = (pass & 1) ? v1 : v0;
