@@ -750,15 +750,16 @@ __global__ __launch_bounds__(256) void k_composite(
   const uint32_t* __restrict__ vals = (plan->passes & 1) ? v1 : v0;
   const int2 rg0 = ranges[gt];  // loaded alongside the flag
   const int2 rg = plan->overflow ? make_int2(-1, -1) : rg0;
+  // A pixel is live while T > kTransmittanceEps: it starts at 1 inside the
+  // image (0 outside, where nothing is written) and the blend that takes it
+  // to kTransmittanceEps or below is the pixel's last.
   float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
   int32_t ne[PIX];  // position of the terminating splat + 1, or the list length
-  bool live[PIX];
 #pragma unroll
   for (int p = 0; p < PIX; ++p) {
     C0[p] = C1[p] = C2[p] = D[p] = 0.0f;
-    T[p] = 1.0f;
+    T[p] = (px < v.width && py0 + p < v.height) ? 1.0f : 0.0f;
     ne[p] = rg.y - rg.x;
-    live[p] = px < v.width && py0 + p < v.height;
   }
   if (rg.x != -1 && rg.x < rg.y) {
     const float pxc = (float)px + 0.5f;
@@ -767,7 +768,7 @@ __global__ __launch_bounds__(256) void k_composite(
     for (int p = 0; p < PIX; ++p) pyc[p] = (float)(py0 + p) + 0.5f;
     bool any = false;
 #pragma unroll
-    for (int p = 0; p < PIX; ++p) any |= live[p];
+    for (int p = 0; p < PIX; ++p) any |= T[p] > kTransmittanceEps;
     for (int base = rg.x; base < rg.y; base += BATCH) {
       if (__syncthreads_count(any) == 0) break;
       const int n = min(BATCH, rg.y - base);
@@ -802,7 +803,10 @@ __global__ __launch_bounds__(256) void k_composite(
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (any) {
+      // every lane walks the list until none of the wave's pixels is live
+      // (a lane whose pixels are done fails their T tests), so the loop's
+      // exit is wave-uniform
+      if (__ballot(any)) {
         for (uint32_t q = 0; q < cnt; ++q) {
           const int j = s_list[w][q];
           const float4 a = s0[j];  // x, y, i00/2, i01
@@ -813,11 +817,11 @@ __global__ __launch_bounds__(256) void k_composite(
           const float txy = a.w * dx;
 #pragma unroll
           for (int p = 0; p < PIX; ++p) {
-            if (!live[p]) continue;
             const float dy = pyc[p] - a.y;
             float power = txx + txy * dy + b.x * dy * dy;
             if (power < 0.0f) power = 0.0f;
-            if (power > b.z) continue;  // power > 4.5, or alpha < 1/255 (power_limit)
+            // a finished pixel, or power > 4.5 or alpha < 1/255 (power_limit)
+            if (!(T[p] > kTransmittanceEps) || power > b.z) continue;
             float alpha = b.y * splat_exp_neg(-power);
             if (alpha > 0.99f) alpha = 0.99f;
             // PixelAccumulator.blend, compositor.mojo:25-53
@@ -829,16 +833,16 @@ __global__ __launch_bounds__(256) void k_composite(
             C2[p] += c.z * wgt;
             D[p] += c.w * wgt;
             T[p] *= (1.0f - alpha);
-            if (T[p] <= kTransmittanceEps) {
-              live[p] = false;
-              ne[p] = base + j - rg.x + 1;
-            }
+            ne[p] = T[p] <= kTransmittanceEps ? base + j - rg.x + 1 : ne[p];
           }
-          any = false;
+          bool alive = false;
 #pragma unroll
-          for (int p = 0; p < PIX; ++p) any |= live[p];
-          if (!any) break;
+          for (int p = 0; p < PIX; ++p) alive |= T[p] > kTransmittanceEps;
+          if (__ballot(alive) == 0) break;
         }
+        any = false;
+#pragma unroll
+        for (int p = 0; p < PIX; ++p) any |= T[p] > kTransmittanceEps;
       }
     }
   }
