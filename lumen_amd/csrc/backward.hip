@@ -126,9 +126,11 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
   if (t == 0) s_last = rg.x - 1;
   __syncthreads();
   int last[PIX];
-  float T[PIX], gC0[PIX], gC1[PIX], gC2[PIX], gD[PIX], Tf[PIX], gsum[PIX];
+  float T[PIX], gC0[PIX], gC1[PIX], gC2[PIX], gD[PIX];
   // Q = sum over channels of dL/dC_k times the colour (and depth) blended
-  // behind the current splat, which is all the opacity gradient needs of it
+  // behind the current splat, which is all the opacity gradient needs of it;
+  // a white background is blended behind every splat (T_final per channel),
+  // so Q starts at sum_k dL/dC_k T_final there
   float Q[PIX], pyc[PIX];
   int lane_last = rg.x - 1;
 #pragma unroll
@@ -147,8 +149,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       gC2[p] = g_rgb[3 * pix + 2];
       gD[p] = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
     }
-    Tf[p] = T[p];
-    gsum[p] = gC0[p] + gC1[p] + gC2[p];
+    if (white) Q[p] = (gC0[p] + gC1[p] + gC2[p]) * T[p];
     lane_last = max(lane_last, last[p]);
   }
   // splats this wave's pixels evaluated: positions [0, wave_cnt) of the list
@@ -183,8 +184,10 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const float lim = power_limit(b.y);
       float hx = c.z, hy = c.w;
       limit_box(lim, hx, hy);
-      s0[j] = a;                                // x, y, i00, 2 i01
-      s1[j] = make_float4(b.x, b.y, lim, 0.0f);  // i11, opacity, power limit
+      // the conic halved as the forward stages it, so the power below is the
+      // forward's expression itself
+      s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
+      s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
       s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
       s_mask[j] = (uint8_t)block_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
                                       tx * v.tile_w, ty * v.tile_h, qw, qh, nwx, nwaves);
@@ -206,8 +209,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int q = (int)cnt - 1; q >= 0; --q) {  // back to front; every lane runs every q
       const int j = s_list[wv][q];
-      const float4 a = s0[j];  // x, y, i00, 2 i01
-      const float4 b = s1[j];  // i11, opacity, power limit
+      const float4 a = s0[j];  // x, y, i00/2, i01
+      const float4 b = s1[j];  // i11/2, opacity, power limit
       const float4 c = s2[j];  // r, g, b, depth
       const float dx = pxc - a.x;
       const float txx = a.z * dx * dx;
@@ -221,40 +224,34 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         // the splat did not blend (the branches below only set these three,
         // so the ten gradient components are formed once, after them)
         float wm = 0.0f, g5 = 0.0f, dpm = 0.0f;
-        if (start + j <= last[p]) {
-          float power = 0.5f * (txx + txy * dy + b.x * dy * dy);
-          bool pclamp = false;
-          if (power < 0.0f) {
-            power = 0.0f;
-            pclamp = true;
-          }
-          if (!(power > b.z)) {  // power_limit: the forward skipped the others too
-            const float G = splat_exp_neg(-power);
-            float alpha = b.y * G;
-            bool aclamp = false;
-            if (alpha > 0.99f) {
-              alpha = 0.99f;
-              aclamp = true;
-            }
-            if (!(alpha < kMinAlpha)) {
-              contrib = true;
-              const float om = 1.0f - alpha;
-              const float rom = __builtin_amdgcn_rcpf(om);
-              const float Tj = T[p] * rom;
-              const float w = alpha * Tj;
-              // dL/dalpha = sum_k g_k (c_k Tj - S_k / (1 - alpha)), S_k the
-              // channel blended behind; sum_k g_k S_k is carried as Q
-              const float gc = gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z + gD[p] * c.w;
-              float dal = gc * Tj - Q[p] * rom;
-              if (white) dal -= gsum[p] * Tf[p] * rom;
-              Q[p] += gc * w;
-              T[p] = Tj;
-              wm = w;
-              if (!aclamp) {
-                g5 = dal * G;
-                if (!pclamp) dpm = -dal * b.y * G;
-              }
-            }
+        float power = txx + txy * dy + b.x * dy * dy;
+        const bool pclamp = power < 0.0f;
+        if (pclamp) power = 0.0f;
+        // splats up to the pixel's last evaluated one, within the power
+        // limit (the forward skipped the others too); one branch, the
+        // clamps' cases below are selects
+        if (start + j <= last[p] && !(power > b.z)) {
+          const float G = splat_exp_neg(-power);
+          float alpha = b.y * G;
+          const bool aclamp = alpha > 0.99f;
+          if (aclamp) alpha = 0.99f;
+          if (!(alpha < kMinAlpha)) {
+            contrib = true;
+            const float om = 1.0f - alpha;
+            const float rom = __builtin_amdgcn_rcpf(om);
+            const float Tj = T[p] * rom;
+            const float w = alpha * Tj;
+            // dL/dalpha = sum_k g_k (c_k Tj - S_k / (1 - alpha)), S_k the
+            // channel blended behind; sum_k g_k S_k is carried as Q
+            const float gc = gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z + gD[p] * c.w;
+            const float dal = gc * Tj - Q[p] * rom;
+            Q[p] += gc * w;
+            T[p] = Tj;
+            wm = w;
+            const float g5v = dal * G;
+            const float dpmv = -dal * b.y * G;
+            g5 = aclamp ? 0.0f : g5v;
+            dpm = (aclamp || pclamp) ? 0.0f : dpmv;
           }
         }
         // the power's gradient as moments of the pixel offset: sum dpm dx,
@@ -292,7 +289,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     __syncthreads();
     for (int j = t; j < n; j += nthreads) {
       float* m = &s_part[0][j * kGradVis];
-      const float i00 = s0[j].z, i01 = 0.5f * s0[j].w, i11 = s1[j].x;
+      const float i00 = 2.0f * s0[j].z, i01 = s0[j].w, i11 = 2.0f * s1[j].x;
       // dL/d(mean) = -Sigma^-1 sum dpm d, dL/d(i00, i01, i11) = (1/2, 1, 1/2) second moments
       const float g[kGradVis] = {-(i00 * m[0] + i01 * m[1]), -(i01 * m[0] + i11 * m[1]),
                                  0.5f * m[2], m[3], 0.5f * m[4], m[5], m[6], m[7], m[8], m[9]};
