@@ -811,6 +811,7 @@ __global__ __launch_bounds__(256) void k_composite(
           const int j = s_list[w][q];
           const float4 a = s0[j];  // x, y, i00/2, i01
           const float4 b = s1[j];  // i11/2, opacity, power limit
+          const float4 c = s2[j];  // r, g, b, depth
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
           const float txx = a.z * dx * dx;
@@ -826,7 +827,6 @@ __global__ __launch_bounds__(256) void k_composite(
             if (alpha > 0.99f) alpha = 0.99f;
             // PixelAccumulator.blend, compositor.mojo:25-53
             if (alpha < kMinAlpha) continue;
-            const float4 c = s2[j];  // r, g, b, depth
             const float wgt = alpha * T[p];
             C0[p] += c.x * wgt;
             C1[p] += c.y * wgt;
