@@ -36,11 +36,17 @@ second timed loop over the same frames.  The roofline entry is the
 stage with the largest share; its algorithmic work per launch is:
   count      68 B / Gaussian (44 B geometry read, 8 B packed count written,
              8 B read + 8 B written by the compaction scan) + 32 B / visible
-             splat (geometry cached for emit)
+             splat (geometry cached for emit); + 4 B / Gaussian when emit
+             runs the first sort pass (its digit histogram, 512 words per
+             512 Gaussians)
   emit       24 B / Gaussian (position, opacity, scan offset) + 296 B / visible
              splat (32 B cached geometry + 192 B SH read, 48 B record + 24 B
-             index, radius and key metadata written) + 12 B / key
-  sort       8 B / key (one histogram read) + 24 B / key per digit pass
+             index, radius and key metadata written) + 8 B / key written
+             (+ 4 B / Gaussian of digit prefixes read when it runs the first
+             sort pass: it then writes the keys in that pass's order)
+  sort       per digit pass the sort runs (the frame's passes less the one
+             emit ran): 4 B / key histogram read + 16 B / key scatter read
+             and write (32-bit keys; 8 + 24 B with 64-bit keys)
   ranges     8 B / key + 8 B / tile
   composite  30 FP32 operations per (pixel, splat) evaluation of the
              reference's compositor (E of them per frame, counted by the
@@ -90,15 +96,16 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def stage_work(N, V, M, T, passes, E, kb):
+def stage_work(N, V, M, T, passes, E, kb, emit_passes=0):
     """Algorithmic bytes (or flops) per launch of each stage; kb is the
-    width of the sorted keys (4 for 32-bit frame keys, 8 for 64-bit)."""
+    width of the sorted keys (4 for 32-bit frame keys, 8 for 64-bit);
+    emit_passes of the passes ran in emit."""
     return {
-        "count": ("hbm", 68.0 * N + 32.0 * V),
-        "emit": ("hbm", 24.0 * N + 296.0 * V + (kb + 4.0) * M),
-        # per pass: the histogram reads the keys, the scatter reads and
-        # writes keys and values
-        "sort": ("hbm", passes * (kb + 2.0 * (kb + 4.0)) * M),
+        "count": ("hbm", 68.0 * N + 32.0 * V + 4.0 * N * emit_passes),
+        "emit": ("hbm", 24.0 * N + 296.0 * V + (kb + 4.0) * M + 4.0 * N * emit_passes),
+        # per pass the sort runs: the histogram reads the keys, the scatter
+        # reads and writes keys and values
+        "sort": ("hbm", (passes - emit_passes) * (kb + 2.0 * (kb + 4.0)) * M),
         "ranges": ("hbm", kb * M + 8.0 * T),
         "composite": ("valu", float(COMPOSITE_FLOPS_PER_EVAL) * E),
     }
@@ -170,6 +177,7 @@ def headline(args, result):
     E = int(R.stage("n_eval").astype(np.int64).sum())
     V, M, passes = out.visible_splats, out.total_splat_instances, out.sort_passes
     kb = out.key_bytes
+    ep = out.emit_passes
     tx, ty = cfg.tiles(W, H)
     T = tx * ty
     N = scene.count
@@ -204,14 +212,14 @@ def headline(args, result):
     ms_events = (te1 - te0) * 1e3 / args.steps
 
     traffic = measured_traffic()
-    work = stage_work(N, V, M, T, passes, E, kb)
+    work = stage_work(N, V, M, T, passes, E, kb, ep)
     stages = {}
     for k, (b, w) in work.items():
         if stage_ms.get(k, 0) > 0:
             kern = KERNELS[k]
-            if k == "sort":  # every pass runs a histogram and a scatter
+            if k == "sort":  # every pass it runs is a histogram and a scatter
                 th, ts = traffic.get("k_sort_hist"), traffic.get("k_sort_scatter")
-                tr = passes * (th + ts) if th and ts else None
+                tr = (passes - ep) * (th + ts) if th and ts else None
             else:
                 tr = traffic.get(kern)
             stages[k] = roofline_entry(b, w, stage_ms[k], int(tr) if tr else None)
@@ -234,7 +242,7 @@ def headline(args, result):
         "config": {"workload": args.workload, "gaussians": N, "width": W, "height": H,
                    "tile": f"{cfg.tile_width}x{cfg.tile_height}", "sh_degree": cfg.max_sh_degree,
                    "mode": "forward", "visible": V, "instances": M, "evaluations": E,
-                   "sort_passes": passes, "sort_key_bytes": kb},
+                   "sort_passes": passes, "sort_passes_in_emit": ep, "sort_key_bytes": kb},
         "roofline": roofline,
         "stages": stages,
         "ms_per_step_with_stage_events": round(ms_events, 4),

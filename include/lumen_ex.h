@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define LUMEN_EX_ABI_VERSION 1
+#define LUMEN_EX_ABI_VERSION 2
 
 /* Status values beyond LumenStatus (c_api/include/lumen.h:29-36). */
 #define LUMEN_EX_ERROR_DEVICE ((LumenStatus)-100)   /* a HIP call failed */
@@ -92,6 +92,10 @@ typedef struct {
     uint64_t tiles_y;
     uint32_t sort_passes;           /* radix digit passes actually run */
     uint32_t key_bytes;             /* 4: 32-bit frame keys were sorted, 8: 64-bit keys */
+    uint32_t emit_passes;           /* of those, passes run while the keys were emitted
+                                       (0 or 1: the first digit, the depth key's low 9 bits,
+                                       on single-view frames of at least 2048 tiles) */
+    uint32_t reserved;
 } LumenExRenderStats;
 
 LumenExContext* lumen_ex_context_create(int device);

@@ -83,7 +83,8 @@ class LumenExRenderConfig(ctypes.Structure):
 class LumenExRenderStats(ctypes.Structure):
     _fields_ = [("visible_splats", ctypes.c_uint64), ("total_splat_instances", ctypes.c_uint64),
                 ("tiles_x", ctypes.c_uint64), ("tiles_y", ctypes.c_uint64),
-                ("sort_passes", ctypes.c_uint32), ("key_bytes", ctypes.c_uint32)]
+                ("sort_passes", ctypes.c_uint32), ("key_bytes", ctypes.c_uint32),
+                ("emit_passes", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class LumenExViewStats(ctypes.Structure):

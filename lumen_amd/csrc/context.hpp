@@ -107,6 +107,10 @@ struct LumenExContext {
   lumen::DevBuf vmeta;    // per visible splat: first key offset, tile rectangle (uint4)
   lumen::DevBuf keys[2], vals[2], keys_keep, vals_keep;
   lumen::DevBuf hist, hist_part;
+  // single-view frames whose first sort pass runs in k_emit: per (digit,
+  // count block) tile counts of k_emit's digit, their exclusive prefixes over
+  // the blocks and the per-digit totals
+  lumen::DevBuf etab, epre, etot;
   lumen::DevBuf ranges, t_final, n_eval, minmax;
   lumen::DevBuf tile_order;  // the compositors' tile order: heaviest key count class first
   lumen::DevBuf partial;   // backward: per tile instance (emission order) 10 partial gradients
@@ -134,6 +138,7 @@ struct LumenExContext {
   bool reserved_set = false;
   uint64_t reserved_v = 0, reserved_m = 0;
   uint32_t key_bytes = 8;         // width of the last plain frame's sort keys
+  uint32_t emit_passes = 0;       // of its passes, those k_emit ran (plan_frame)
   bool sizes_known = false;        // V and M were read back (not a graph replay)
   bool kept = false, pixels = false, has_grad = false;
   GraphCache graph;

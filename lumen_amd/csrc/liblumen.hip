@@ -64,6 +64,7 @@ __global__ void k_init_frame(uint32_t* __restrict__ bounds, SortPlan* __restrict
   bounds[3] = 0u;
   *reinterpret_cast<uint64_t*>(bounds + 4) = 0ull;  // exact instance total
   plan->passes = 0;
+  plan->first = 0;
   plan->m = 0;
   plan->v = 0;
   plan->overflow = 0;
@@ -96,25 +97,42 @@ struct GeoCache {
 // INL: the views come in the kernel's arguments (iv) instead of device
 // memory, and block 0 writes them to views_out for the frame's later kernels
 // (a frame of at most kInlineViews views needs no copy of its own).
+// With etab (single-view frames, frame_fusable), each block also writes the
+// histogram of k_emit's digit (depth key & 511) over its Gaussians, weighted
+// by their tile counts: etab[digit * blocks + block], the rows k_emit's first
+// sort pass needs.  Such frames count in blocks of kFusedBlock Gaussians (a
+// half of the table rows 256 would give: fewer partial-line writes here and
+// shorter column scans in k_plan).  Grid block i takes Gaussian
+// block xcd_block(i) (k_emit uses the same numbering), so the neighbouring
+// blocks that share a line of the table run on one XCD.
 constexpr int kInlineViews = 4;
+constexpr int kFusedBlock = 512;
 struct InlineViews {
   ViewParams v[kInlineViews];
 };
 static_assert(sizeof(ViewParams) % 4 == 0 && kInlineViews * sizeof(ViewParams) / 4 <= 256,
               "block 0 copies the views a word per thread");
-template <bool INL>
-__global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __restrict__ views,
+template <bool INL, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* __restrict__ views,
                                                int nviews, uint64_t* __restrict__ packed,
                                                GeoCache* __restrict__ geo,
                                                uint4* __restrict__ block_bounds,
                                                uint64_t* __restrict__ block_inst, InlineViews iv,
-                                               ViewParams* __restrict__ views_out) {
-  __shared__ uint32_t sb[4][4];
-  __shared__ uint64_t sinst[4];
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+                                               ViewParams* __restrict__ views_out,
+                                               uint32_t* __restrict__ etab) {
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint32_t sb[NW][4];
+  __shared__ uint64_t sinst[NW];
+  __shared__ uint32_t eh[kDigitBins];
+  const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);  // the block's Gaussians
+  const uint32_t i = gb * (uint32_t)BLOCK + threadIdx.x;
   if (INL && blockIdx.x == 0 && threadIdx.x < nviews * (int)(sizeof(ViewParams) / 4))
     reinterpret_cast<uint32_t*>(views_out)[threadIdx.x] =
         reinterpret_cast<const uint32_t*>(iv.v)[threadIdx.x];
+  if (etab) {  // uniform: a kernel argument
+    for (int e = threadIdx.x; e < kDigitBins; e += BLOCK) eh[e] = 0u;
+    __syncthreads();
+  }
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
   uint64_t inst = 0ull;
   if (i < s.n) {
@@ -133,6 +151,7 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
         val = ((uint64_t)cnt << 32) | 1ull;
         inst += cnt;
         const uint32_t dk = depth_key(g.depth);
+        if (etab && vw == 0) atomicAdd(&eh[dk & kEmitDigitMask], cnt);
         dmin = min(dmin, dk);
         dmax = max(dmax, dk);
         tmin = min(tmin, (uint32_t)(g.min_ty * v.tiles_x + g.min_tx));  // view-local tiles
@@ -162,14 +181,20 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < 4; ++k) {
+    uint64_t bi = sinst[0];
+    for (int k = 1; k < NW; ++k) {
       dmin = min(dmin, sb[k][0]);
       dmax = max(dmax, sb[k][1]);
       tmin = min(tmin, sb[k][2]);
       tmax = max(tmax, sb[k][3]);
+      bi += sinst[k];
     }
     block_bounds[blockIdx.x] = make_uint4(dmin, dmax, tmin, tmax);
-    block_inst[blockIdx.x] = sinst[0] + sinst[1] + sinst[2] + sinst[3];
+    block_inst[blockIdx.x] = bi;
+  }
+  if (etab) {  // the __syncthreads above ordered the LDS atomics
+    for (int e = threadIdx.x; e < kDigitBins; e += BLOCK)
+      etab[(size_t)e * gridDim.x + gb] = eh[e];
   }
 }
 
@@ -182,16 +207,50 @@ __global__ __launch_bounds__(256) void k_count(SceneDev s, const ViewParams* __r
 // nothing and the frame renders background.  With host_ret (mapped, coherent
 // host memory) block 0 also writes the scan total, the four bounds and the
 // exact instance total there, the 32 bytes a plain frame sizes itself by.
+// With etab (k_count wrote k_emit's digit histograms, one row per count
+// block), plan_frame may let k_emit run the first pass, and blocks 1 to 512
+// each scan one digit's column of the table over the count blocks into epre
+// (exclusive prefixes; out of place, so a frame that enqueues k_plan again
+// after growing its buffers scans the same counts) and its total into etot.
+constexpr int kPlanScanItems = 4;
 __global__ __launch_bounds__(1024) void k_plan(
     const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ totals,
     const uint4* __restrict__ block_bounds, const uint64_t* __restrict__ block_inst,
     uint32_t nblocks, uint32_t* __restrict__ bounds, uint32_t n, int nviews, uint64_t m_cap,
     uint64_t v_cap, SortPlan* __restrict__ plan, ViewSpan* __restrict__ spans,
     SortSeg* __restrict__ segs, int2* __restrict__ ranges, uint32_t tiles, int force_items,
+    const uint32_t* __restrict__ etab, uint32_t* __restrict__ epre, uint32_t* __restrict__ etot,
     uint64_t* __restrict__ host_ret) {
   // every block: empty tile ranges (-1, -1) before k_ranges fills the used ones
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
     ranges[t] = make_int2(-1, -1);
+  const bool fuse = etab != nullptr;
+  if (fuse && blockIdx.x >= 1 && blockIdx.x <= (uint32_t)kDigitBins) {
+    __shared__ uint32_t s_w[16];
+    const uint32_t d = blockIdx.x - 1u;
+    const uint32_t* __restrict__ src = etab + (size_t)d * nblocks;
+    uint32_t* __restrict__ dst = epre + (size_t)d * nblocks;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += 1024u * kPlanScanItems) {  // uniform
+      uint32_t v[kPlanScanItems], sum = 0;
+      const uint32_t j0 = b0 + threadIdx.x * kPlanScanItems;
+#pragma unroll
+      for (int j = 0; j < kPlanScanItems; ++j) {
+        v[j] = j0 + j < nblocks ? src[j0 + j] : 0u;
+        sum += v[j];
+      }
+      uint32_t tot;
+      uint32_t ex = block_exclusive_scan<uint32_t, 1024>(sum, s_w, tot) + carry;
+#pragma unroll
+      for (int j = 0; j < kPlanScanItems; ++j) {
+        if (j0 + j < nblocks) dst[j0 + j] = ex;
+        ex += v[j];
+      }
+      carry += tot;
+    }
+    if (threadIdx.x == 0) etot[d] = carry;
+    return;
+  }
   if (blockIdx.x != 0) return;
   __shared__ uint32_t sb[16][4];
   __shared__ uint64_t sinst[16];
@@ -265,8 +324,9 @@ __global__ __launch_bounds__(1024) void k_plan(
   const uint32_t tile = 256u * (uint32_t)items;
   if (threadIdx.x == 0) {
     SortPlan p;
-    plan_digits(V > 0, s_bounds[0], s_bounds[1], s_bounds[2], s_bounds[3], p);
+    plan_frame(V > 0, s_bounds[0], s_bounds[1], s_bounds[2], s_bounds[3], fuse, p);
     plan->passes = p.passes;
+    plan->first = p.first;
     for (int k = 0; k < 8; ++k) {
       plan->shift[k] = p.shift[k];
       plan->width[k] = p.width[k];
@@ -356,7 +416,24 @@ __device__ __forceinline__ void splat_box(float i00, float i01, float i11, float
 // thread's keys only move forward, so its search continues from the last
 // splat it found (one LDS read while it stays inside a large splat).
 // Emission order within a splat is ty-major then tx (pipeline.mojo:152-157).
-__global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __restrict__ views,
+//
+// When the plan's first pass is k_emit's (plan->first == 1, plan_frame), the
+// keys go to buffer 1 in the order that LSD pass 0 over the depth key's low 9
+// bits would give, instead of emission order.  A splat's keys share that
+// digit and stay together, so each splat's run moves as one: its output start
+// is the view's first key + the digit's start in the view (exclusive scan of
+// the digit's totals over the view's blocks) + the block's prefix in the
+// digit's column (both from k_count's table, scanned in k_plan) +
+// the tile counts of the block's earlier splats with the same digit.  The
+// last term is a wave's earlier lanes with the same digit (found with one
+// ballot per digit bit) plus the earlier waves' digit sums.  Emission order
+// breaks the digit's ties, as the stable pass does.
+// MULTI: more than one view, the Gaussian's SH coefficients loaded once and
+// kept in registers across the views.  FUSABLE: a single-view frame whose
+// k_count wrote the digit table in blocks of BLOCK Gaussians (the plan decides
+// whether the pass runs here); without it the kernel carries none of that code.
+template <bool MULTI, bool FUSABLE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __restrict__ views,
                                               int nviews, const uint64_t* __restrict__ offsets,
                                               const uint64_t* __restrict__ totals,
                                               const GeoCache* __restrict__ geo, uint64_t m_cap,
@@ -366,17 +443,37 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
                                               uint4* __restrict__ vmeta,
                                               uint64_t* __restrict__ keys,
                                               uint32_t* __restrict__ vals,
-                                              SortPlan* __restrict__ plan) {
+                                              SortPlan* __restrict__ plan,
+                                              const uint32_t* __restrict__ etab,
+                                              const uint32_t* __restrict__ etot,
+                                              uint64_t* __restrict__ keys1,
+                                              uint32_t* __restrict__ vals1) {
+  static_assert(!(MULTI && FUSABLE), "the first pass runs in k_emit for single views only");
+  constexpr int NW = BLOCK / 64;
+  // digits per thread for the table row (512 digits over the block's threads)
+  constexpr int DPT = kDigitBins / BLOCK > 0 ? kDigitBins / BLOCK : 1;
   // per thread of the block: (key start - block base, rect min, rect max,
   // depth key) and the visible index
-  __shared__ uint4 s_meta[256];
-  __shared__ uint32_t s_vi[256];
+  __shared__ uint4 s_meta[BLOCK];
+  __shared__ uint32_t s_vi[BLOCK];
+  // first pass in k_emit: each thread's output start, per-wave digit sums
+  // (then their exclusive prefixes over the waves) and the block's start of
+  // each digit in the view's keys
+  __shared__ uint32_t s_dst[FUSABLE ? BLOCK : 1];
+  __shared__ uint32_t s_wh[FUSABLE ? NW : 1][kDigitBins];
+  __shared__ uint32_t s_delta[kDigitBins];
+  __shared__ uint32_t s_w[NW];
   if (plan->overflow) return;  // k_plan: the frame exceeds the capacities
+  const bool fused = FUSABLE && plan->first == 1;  // only ever with etab (k_plan's fuse)
   const int t = threadIdx.x;
-  const uint32_t first = blockIdx.x * 256u;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const uint32_t nblk = gridDim.x;
+  const uint32_t gb = xcd_block(blockIdx.x, nblk);  // as k_count numbers them
+  const uint32_t first = gb * (uint32_t)BLOCK;
   const uint32_t i = first + (uint32_t)t;
   const bool valid = i < s.n;
-  const uint32_t last = min(first + 256u, s.n) - 1u;  // the block's last Gaussian
+  const uint32_t last = min(first + (uint32_t)BLOCK, s.n) - 1u;  // the block's last Gaussian
   const uint64_t nv = (uint64_t)nviews * s.n;
   float px = 0.0f, py = 0.0f, pz = 0.0f, op = 0.0f;
   if (valid) {
@@ -390,9 +487,12 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
   const bool frame_keys = plan->mode == 1;
   const uint32_t kt0 = plan->tmin, kd0 = plan->dmin;
   const int kdb = plan->dbits;
-  uint32_t* __restrict__ keys32 = reinterpret_cast<uint32_t*>(keys);
+  uint64_t* __restrict__ kout = fused ? keys1 : keys;
+  uint32_t* __restrict__ vout = fused ? vals1 : vals;
+  uint32_t* __restrict__ keys32 = reinterpret_cast<uint32_t*>(kout);
   bool have_sh = false;
   float c[48];
+  if (!MULTI) nviews = 1;
   for (int vw = 0; vw < nviews; ++vw) {
     const ViewParams& v = views[vw];
     // the block's runs: keys [base, end), visible splats [vbase, vend)
@@ -405,13 +505,31 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       if (t == 0) atomicOr(&plan->overflow, 1u);
       continue;
     }
-    uint32_t vis = 0, ko = base, vi = 0;
+    // first pass here: this block's row of the digit table (DPT digits per
+    // thread), loaded ahead of the splat work
+    uint32_t gl[DPT], ct[DPT], vkb = 0u;
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) gl[e] = ct[e] = 0u;
+    if (fused) {
+      if (t * DPT < kDigitBins) {
+#pragma unroll
+        for (int e = 0; e < DPT; ++e) {
+          const uint32_t col = (uint32_t)(t * DPT + e);
+          gl[e] = etab[(size_t)col * nblk + gb];
+          ct[e] = etot[col];
+        }
+      }
+      vkb = (uint32_t)(offsets[row] >> 32);  // the view's first key
+      for (int e = t; e < NW * kDigitBins; e += BLOCK) (&s_wh[0][0])[e] = 0u;
+    }
+    uint32_t vis = 0, ko = base, vi = 0, cnt = 0;
     if (valid) {
       const uint64_t off0 = offsets[row + i];
       const uint64_t off1 = row + i + 1 < nv ? offsets[row + i + 1] : *totals;
       vis = (uint32_t)off1 - (uint32_t)off0;
       vi = (uint32_t)off0;
       ko = (uint32_t)(off0 >> 32);
+      cnt = (uint32_t)(off1 >> 32) - ko;
     }
     uint32_t rmin = 0, rmax = 0, dk = 0;
     if (vis) {
@@ -420,7 +538,7 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       const float c00 = gc.a.w, c01 = gc.b.x, c11 = gc.b.y;
       rmin = __float_as_uint(gc.b.z);
       rmax = __float_as_uint(gc.b.w);
-      if (!have_sh) {
+      if (!MULTI || !have_sh) {
         const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
         const int nfloat = 3 * v.sh_coeffs;
 #pragma unroll
@@ -467,12 +585,62 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
     // of equal starts
     s_meta[t] = make_uint4(valid ? ko - base : 0xFFFFFFFFu, rmin, rmax, dk);
     s_vi[t] = vi;
+    if (fused) {
+      // the digit's start in the view: an exclusive scan of its totals
+      uint32_t all, csum = 0;
+#pragma unroll
+      for (int e = 0; e < DPT; ++e) csum += ct[e];
+      uint32_t ex = block_exclusive_scan<uint32_t, BLOCK>(csum, s_w, all);
+      if (t * DPT < kDigitBins) {
+#pragma unroll
+        for (int e = 0; e < DPT; ++e) {
+          s_delta[t * DPT + e] = vkb + ex + gl[e];
+          ex += ct[e];
+        }
+      }
+      // tile counts of this wave's earlier lanes with the same digit (such
+      // lanes are rare: the loop runs as often as the most of them below
+      // any one lane)
+      const bool has = cnt > 0;
+      const uint32_t d = dk & kEmitDigitMask;
+      uint64_t peers = __ballot(has);
+#pragma unroll
+      for (int bit = 0; bit < kEmitDigitBits; ++bit) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bb = __ballot(on);
+        peers &= on ? bb : ~bb;
+      }
+      uint64_t below = has ? (peers & lanemask_lt()) : 0ull;
+      uint32_t r = 0;
+      while (__ballot(below != 0ull)) {  // wave-uniform
+        const int src = below ? __ffsll((unsigned long long)below) - 1 : lane;
+        const uint32_t cs = (uint32_t)__shfl((int)cnt, src, 64);
+        if (below) {
+          r += cs;
+          below &= below - 1ull;
+        }
+      }
+      if (has) atomicAdd(&s_wh[w][d], cnt);
+      __syncthreads();
+      // each digit's sums become the earlier waves' totals
+      for (int dd = t; dd < kDigitBins; dd += BLOCK) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          const uint32_t x = s_wh[k][dd];
+          s_wh[k][dd] = run;
+          run += x;
+        }
+      }
+      __syncthreads();
+      if (has) s_dst[t] = s_delta[d] + s_wh[w][d] + r;
+    }
     __syncthreads();
     const uint32_t total = end - base;
     int g = 0;  // the splat owning key p (thread 0 starts at 0)
-    for (uint32_t p = (uint32_t)t; p < total; p += 256u) {
-      if (g < 255 && s_meta[g + 1].x <= p) {
-        int lo = g + 1, hi = 255;
+    for (uint32_t p = (uint32_t)t; p < total; p += (uint32_t)BLOCK) {
+      if (g < BLOCK - 1 && s_meta[g + 1].x <= p) {
+        int lo = g + 1, hi = BLOCK - 1;
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
           if (s_meta[mid].x <= p) lo = mid;
@@ -488,11 +656,13 @@ __global__ __launch_bounds__(256) void k_emit(SceneDev s, const ViewParams* __re
       const uint32_t ty = min_ty + r;
       const uint32_t tx = min_tx + (local - r * wtiles);
       const uint32_t tile = ty * (uint32_t)v.tiles_x + tx;  // within the view
+      const uint32_t dst = fused ? s_dst[g] + local : base + p;
+      if (fused && dst >= m_cap) continue;  // never with a consistent digit table
       if (frame_keys)
-        keys32[base + p] = ((tile - kt0) << kdb) | (m.w - kd0);
+        keys32[dst] = ((tile - kt0) << kdb) | (m.w - kd0);
       else
-        keys[base + p] = ((uint64_t)tile << 32) | (uint64_t)m.w;
-      vals[base + p] = s_vi[g];
+        kout[dst] = ((uint64_t)tile << 32) | (uint64_t)m.w;
+      vout[dst] = s_vi[g];
     }
     __syncthreads();
   }
@@ -895,6 +1065,7 @@ __global__ void k_set_plan(SortPlan* plan, SortSeg* seg, int passes, const int32
   plan->blocks = seg->nblocks;
   plan->items = items;
   plan->passes = passes;
+  plan->first = 0;
   for (int k = 0; k < 8; ++k) {
     plan->shift[k] = k < passes ? 8 * digits[k] : 0;
     plan->width[k] = k < passes ? 8 : 0;
@@ -958,7 +1129,23 @@ struct FrameShape {
   uint64_t tiles;   // frame tiles (sum over views)
   uint64_t pixels;  // frame pixels (sum over views)
   bool inline_views = false;  // k_count takes the views as arguments and writes the device copy
+  // k_count writes per-block histograms of k_emit's digit, so the sort's
+  // first pass can run inside k_emit (frame_fusable)
+  bool fusable = false;
 };
+
+// Frames whose first sort pass may run inside k_emit: one view of at least
+// 2048 tiles.  The per-block histograms cost about 8 bytes per Gaussian
+// (written, scanned, read), against 20 bytes per key for a sort pass; frames
+// of smaller images hold few keys per Gaussian, and their passes are short.
+// A frame that keeps its unsorted keys (LUMEN_EX_KEEP_STAGES) emits them in
+// the reference's order instead.
+static bool frame_fusable(uint32_t nviews, uint64_t tiles, uint64_t n, bool keep) {
+  return !keep && n > 0 && nviews == 1 && tiles >= 2048ull;
+}
+
+// Gaussians per block of k_count and k_emit (the digit table's rows).
+static uint32_t count_block(const FrameShape& fs) { return fs.fusable ? kFusedBlock : 256u; }
 
 // Per-view parameters on the host and their device copy: through pinned
 // staging, ordered on the stream, or (allow_inline, at most kInlineViews
@@ -1067,6 +1254,11 @@ static LumenStatus frame_buffers(LumenExContext* c, Need&& need, uint64_t N, con
   LUMEN_TRY(need(c->plan, sizeof(SortPlan)));
   LUMEN_TRY(need(c->ranges, fs.tiles * 8));
   LUMEN_TRY(need(c->tile_order, fs.tiles * 4));
+  if (fs.fusable) {
+    LUMEN_TRY(need(c->etab, (uint64_t)kDigitBins * div_up(N, kFusedBlock) * 4));
+    LUMEN_TRY(need(c->epre, (uint64_t)kDigitBins * div_up(N, kFusedBlock) * 4));
+    LUMEN_TRY(need(c->etot, (uint64_t)kDigitBins * 4));
+  }
   if (save_pixels) {
     LUMEN_TRY(need(c->t_final, fs.pixels * 4));
     LUMEN_TRY(need(c->n_eval, fs.pixels * 4));
@@ -1118,21 +1310,20 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
     return LUMEN_SUCCESS;
   }
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, false, st));
-  const uint32_t nblocks = div_up(sd.n, 256);
+  const uint32_t cb = count_block(fs);
+  const uint32_t nblocks = div_up(sd.n, cb);
   uint4* bb = c->block_bounds.as<uint4>();
   uint64_t* binst = reinterpret_cast<uint64_t*>(bb + nblocks);
   InlineViews iv;
   memset(&iv, 0, sizeof(iv));
-  if (fs.inline_views) {
-    memcpy(iv.v, c->host_views.data(), sizeof(ViewParams) * fs.nviews);
-    hipLaunchKernelGGL(k_count<true>, dim3(nblocks), dim3(256), 0, st, sd,
-                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
-                       c->geo.as<GeoCache>(), bb, binst, iv, c->views.as<ViewParams>());
-  } else {
-    hipLaunchKernelGGL(k_count<false>, dim3(nblocks), dim3(256), 0, st, sd,
-                       c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
-                       c->geo.as<GeoCache>(), bb, binst, iv, nullptr);
-  }
+  uint32_t* etab = fs.fusable ? c->etab.as<uint32_t>() : nullptr;
+  if (fs.inline_views) memcpy(iv.v, c->host_views.data(), sizeof(ViewParams) * fs.nviews);
+  ViewParams* vout = fs.inline_views ? c->views.as<ViewParams>() : nullptr;
+  auto count = fs.fusable ? (fs.inline_views ? k_count<true, kFusedBlock> : k_count<false, kFusedBlock>)
+                          : (fs.inline_views ? k_count<true, 256> : k_count<false, 256>);
+  hipLaunchKernelGGL(count, dim3(nblocks), dim3(cb), 0, st, sd, c->views.as<ViewParams>(),
+                     (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(), bb, binst,
+                     iv, vout, etab);
   LUMEN_LAUNCHED();
   const uint64_t nv = (uint64_t)fs.nviews * sd.n;
   LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
@@ -1151,15 +1342,18 @@ static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const Fra
                                 hipStream_t st, bool read_back = false) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0) {
-    const uint32_t nblocks = div_up(sd.n, 256);
+    const uint32_t nblocks = div_up(sd.n, count_block(fs));  // k_count's blocks
     const uint4* bb = c->block_bounds.as<uint4>();
     const uint64_t* binst = reinterpret_cast<const uint64_t*>(bb + nblocks);
-    const uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 1024 * 16);
+    uint32_t pb = div_up(fs.tiles ? fs.tiles : 1, 1024 * 16);
+    if (fs.fusable && pb < 1u + kDigitBins) pb = 1u + kDigitBins;  // the column scans
     hipLaunchKernelGGL(k_plan, dim3(pb), dim3(1024), 0, st, c->packed.as<uint64_t>(),
                        c->totals.as<uint64_t>(), bb, binst, nblocks, frame_bounds(c), sd.n,
                        (int)fs.nviews, m_cap, v_cap, plan, c->spans.as<ViewSpan>(),
                        c->segs.as<SortSeg>(), c->ranges.as<int2>(), (uint32_t)fs.tiles,
-                       force_items, read_back ? c->ret_dev : nullptr);
+                       force_items, fs.fusable ? c->etab.as<uint32_t>() : nullptr,
+                       fs.fusable ? c->epre.as<uint32_t>() : nullptr,
+                       fs.fusable ? c->etot.as<uint32_t>() : nullptr, read_back ? c->ret_dev : nullptr);
     LUMEN_LAUNCHED();
     if (read_back) {
       if (!c->ev_totals) LUMEN_HIP(hipEventCreateWithFlags(&c->ev_totals, hipEventDisableTiming));
@@ -1171,12 +1365,19 @@ static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const Fra
   }
   if (sd.n > 0 && m_cap > 0) {
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, false, st));
-    hipLaunchKernelGGL(k_emit, dim3(div_up(sd.n, 256)), dim3(256), 0, st, sd,
+    // (a frame that cannot run the first pass here keeps the SH coefficients
+    // across views whatever their number: measured faster at one view too)
+    const uint32_t eb = count_block(fs);
+    auto emit = fs.fusable ? k_emit<false, true, kFusedBlock> : k_emit<true, false, 256>;
+    hipLaunchKernelGGL(emit, dim3(div_up(sd.n, eb)), dim3(eb), 0, st, sd,
                        c->views.as<ViewParams>(), (int)fs.nviews, c->packed.as<uint64_t>(),
                        c->totals.as<uint64_t>(), c->geo.as<GeoCache>(), m_cap,
                        v_cap, c->rec.as<float4>(), c->vis2orig.as<int32_t>(),
                        c->radius.as<int32_t>(), c->vmeta.as<uint4>(), c->keys[0].as<uint64_t>(),
-                       c->vals[0].as<uint32_t>(), plan);
+                       c->vals[0].as<uint32_t>(), plan,
+                       fs.fusable ? c->epre.as<uint32_t>() : nullptr,
+                       fs.fusable ? c->etot.as<uint32_t>() : nullptr, c->keys[1].as<uint64_t>(),
+                       c->vals[1].as<uint32_t>());
     LUMEN_LAUNCHED();
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_EMIT, true, st));
   }
@@ -1185,9 +1386,9 @@ static LumenStatus enqueue_emit(LumenExContext* c, const SceneDev& sd, const Fra
 
 // Stage 3 of a frame: sort, ranges and compositing of at most m_cap keys.
 static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const FrameShape& fs,
-                                  const LumenExRenderConfig* cfg, uint64_t m_cap, int sort_slots,
-                                  int sort_modes, int items, bool keep, float* out_rgb,
-                                  float* out_depth, hipStream_t st) {
+                                  const LumenExRenderConfig* cfg, uint64_t m_cap, int sort_first,
+                                  int sort_slots, int sort_modes, int items, bool keep,
+                                  float* out_rgb, float* out_depth, hipStream_t st) {
   SortPlan* plan = c->plan.as<SortPlan>();
   if (sd.n > 0 && m_cap > 0) {
     if (keep) {
@@ -1202,7 +1403,7 @@ static LumenStatus enqueue_finish(LumenExContext* c, const SceneDev& sd, const F
     uint32_t* vb[2] = {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()};
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, false, st));
     LUMEN_TRY(radix_sort_planned(kb, vb, plan, c->segs.as<SortSeg>(), (int)fs.nviews, m_cap,
-                                 sort_slots, sort_modes, items,
+                                 sort_first, sort_slots, sort_modes, items,
                                  SortScratch{&c->hist, &c->hist_part}, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_SORT, true, st));
     LUMEN_TRY(mark(c, LUMEN_EX_TIME_RANGES, false, st));
@@ -1233,7 +1434,7 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
                                 float* out_depth, hipStream_t st) {
   const int items = sort_items_for(m_cap);
   LUMEN_TRY(enqueue_emit(c, sd, fs, v_cap, m_cap, items, st));
-  return enqueue_finish(c, sd, fs, cfg, m_cap, sort_slots, sort_modes, items, keep, out_rgb,
+  return enqueue_finish(c, sd, fs, cfg, m_cap, 0, sort_slots, sort_modes, items, keep, out_rgb,
                         out_depth, st);
 }
 
@@ -1287,6 +1488,8 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st, /*allow_inline=*/sc->count > 0));
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+  const bool keep_flag = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0;
+  fs.fusable = frame_fusable(nviews, fs.tiles, sc->count, keep_flag);
   LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
   const SceneDev sd = scene_dev(sc);
   for (bool& r : c->stage_ran) r = false;
@@ -1330,11 +1533,11 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   // the plan k_plan will write, known here from the bounds: only its passes
   // and key width are enqueued
   SortPlan hp;
-  plan_digits(V > 0, hb[0], hb[1], hb[2], hb[3], hp);
+  plan_frame(V > 0, hb[0], hb[1], hb[2], hb[3], fs.fusable, hp);
   const int np = hp.passes;
-  const bool keep = (cfg->flags & LUMEN_EX_KEEP_STAGES) != 0 && M > 0;
-  LUMEN_TRY(enqueue_finish(c, sd, fs, cfg, M, np, hp.mode == 1 ? 2 : 1, sort_items_for(M), keep,
-                           out_rgb, out_depth, st));
+  const bool keep = keep_flag && M > 0;
+  LUMEN_TRY(enqueue_finish(c, sd, fs, cfg, M, hp.first, np, hp.mode == 1 ? 2 : 1,
+                           sort_items_for(M), keep, out_rgb, out_depth, st));
   c->V = V;
   c->M = M;
   c->T = fs.tiles;
@@ -1346,6 +1549,7 @@ static LumenStatus render_plain(LumenExContext* c, const LumenExScene* sc,
   c->v_cap = V;
   c->m_cap = M;
   c->key_bytes = hp.mode == 1 ? 4u : 8u;
+  c->emit_passes = (uint32_t)hp.first;
   c->sizes_known = true;
   c->kept = keep;
   c->pixels = save;
@@ -1382,6 +1586,7 @@ static LumenStatus render_reserved(LumenExContext* c, const LumenExScene* sc,
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st, /*allow_inline=*/sc->count > 0));
   const bool save = (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0;
+  fs.fusable = frame_fusable(nviews, fs.tiles, sc->count, false);
   LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, save));
   const int items = sort_items_for(m_now);
   LUMEN_TRY(ensure(c->hist, sort_hist_len(m_now, (int)nviews, items) * sizeof(uint32_t)));
@@ -1459,6 +1664,8 @@ static LumenStatus render_graph(LumenExContext* c, const LumenExScene* sc,
   LUMEN_TRY(ensure_capacity(c, v_cap, m_cap));
   FrameShape fs;
   LUMEN_TRY(upload_views(c, cams, nviews, cfg, fs, st));
+  fs.fusable = frame_fusable(nviews, fs.tiles, sc->count, false);
+  LUMEN_TRY(ensure_frame_buffers(c, sc->count, fs, (cfg->flags & LUMEN_EX_SAVE_PIXELS) != 0));
   // sort scratch for the capacity, allocated before capture
   const int items = sort_items_for(m_cap);
   LUMEN_TRY(ensure(c->hist, sort_hist_len(m_cap, (int)nviews, items) * sizeof(uint32_t)));
@@ -1506,7 +1713,7 @@ static std::vector<DevBuf*> context_buffers(LumenExContext* c) {
   return {&c->packed, &c->part,     &c->totals,    &c->plan,      &c->views,    &c->spans,
           &c->geo,    &c->block_bounds, &c->rec,   &c->vis2orig,  &c->radius,   &c->vmeta,
           &c->keys[0], &c->keys[1], &c->vals[0],   &c->vals[1],   &c->keys_keep, &c->vals_keep,
-          &c->hist,   &c->hist_part, &c->ranges, &c->tile_order,   &c->t_final,   &c->n_eval,   &c->minmax,
+          &c->hist,   &c->hist_part, &c->etab, &c->epre, &c->etot, &c->ranges, &c->tile_order,   &c->t_final,   &c->n_eval,   &c->minmax,
           &c->partial, &c->grad_vis, &c->sum_seg, &c->sum_aux, &c->loss_part, &c->dens_cls,
           &c->dens_cnt, &c->dens_off, &c->dens_misc, &c->segs};
 }
@@ -1551,7 +1758,9 @@ static LumenStatus workspace(LumenExContext* c, Need&& need, uint64_t N, const W
   LUMEN_TRY(need(c->views, sizeof(ViewParams) * ws.fs.nviews));
   LUMEN_TRY(need(c->spans, sizeof(ViewSpan) * ws.fs.nviews));
   LUMEN_TRY(need(c->segs, sizeof(SortSeg) * ws.fs.nviews));
-  LUMEN_TRY(frame_buffers(c, need, N, ws.fs, save));
+  FrameShape fs = ws.fs;
+  fs.fusable = frame_fusable(fs.nviews, fs.tiles, N, (flags & LUMEN_EX_KEEP_STAGES) != 0);
+  LUMEN_TRY(frame_buffers(c, need, N, fs, save));
   LUMEN_TRY(capacity_buffers(c, need, v_cap, m_cap));
   if (m_cap > 0) {
     // the larger table of the two tile sizes a frame of up to m_cap keys may use
@@ -1724,6 +1933,8 @@ LumenStatus lumen_ex_render(LumenExContext* c, const LumenExScene* sc, const Lum
     stats->tiles_y = (uint64_t)c->host_views[0].tiles_y;
     stats->sort_passes = (uint32_t)passes;
     stats->key_bytes = c->key_bytes;
+    stats->emit_passes = c->emit_passes;
+    stats->reserved = 0;
   }
   return LUMEN_SUCCESS;
 }
@@ -1821,8 +2032,8 @@ LumenStatus lumen_ex_radix_sort(LumenExContext* c, uint64_t* keys, uint32_t* val
   LUMEN_LAUNCHED();
   uint64_t* kb[2] = {keys, c->keys[1].as<uint64_t>()};
   uint32_t* vb[2] = {values, c->vals[1].as<uint32_t>()};
-  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), c->segs.as<SortSeg>(), 1, n, nd,
-                               1, items, SortScratch{&c->hist, &c->hist_part}, st));
+  LUMEN_TRY(radix_sort_planned(kb, vb, c->plan.as<SortPlan>(), c->segs.as<SortSeg>(), 1, n, 0,
+                               nd, 1, items, SortScratch{&c->hist, &c->hist_part}, st));
   if (nd & 1) {
     LUMEN_HIP(hipMemcpyAsync(keys, kb[1], n * 8, hipMemcpyDeviceToDevice, st));
     LUMEN_HIP(hipMemcpyAsync(values, vb[1], n * 4, hipMemcpyDeviceToDevice, st));

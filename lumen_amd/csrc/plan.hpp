@@ -44,6 +44,8 @@ struct SortPlan {
   int32_t dbits;       // depth bits of a frame key (mode 1)
   uint32_t blocks;     // sort blocks over all segments (the histogram table's rows / 512)
   int32_t items;       // keys per thread of a sort block (sort_items_for)
+  int32_t first;       // passes run before the sort: 1 when k_emit wrote its keys
+                       // already ordered by pass 0's digit (see plan_frame), else 0
 };
 
 // One segment of a frame's keys: a view's contiguous run (emission is
@@ -105,6 +107,7 @@ __host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dm
 #pragma unroll
   for (int k = 0; k < 8; ++k) p.shift[k] = p.width[k] = 0;
   p.passes = 0;
+  p.first = 0;
   p.mode = 1;
   p.tmin = tmin;
   p.dmin = dmin;
@@ -136,6 +139,53 @@ __host__ __device__ inline void plan_digits(bool any, uint32_t dmin, uint32_t dm
     if (k < nd + nt) {
       p.shift[k] = 8 * (k < nd ? k : 4 + (k - nd));
       p.width[k] = 8;
+    }
+  }
+}
+
+// The digit k_emit sorts by when it runs a frame's first pass itself: the
+// depth key's low 9 bits.  All keys of one splat share their depth key, so
+// k_count can histogram this digit per block of Gaussians from the splats'
+// tile counts, and k_emit can write each splat's run of keys straight to its
+// place in the pass's output (pass 0 of an LSD sort over these bits).
+constexpr int kEmitDigitBits = kMaxDigitBits;
+constexpr uint32_t kEmitDigitMask = (1u << kEmitDigitBits) - 1u;
+
+// The frame's sort: plan_digits, or, when `fuse` (the frame's k_count wrote
+// the per-block digit histograms) and it saves a pass, the plan whose pass 0
+// k_emit runs (first = 1).  That plan keeps 32-bit frame keys with the depth
+// offset rounded down to a multiple of 512 and at least 9 depth bits, so key
+// bits 0-8 are exactly (depth key & 511); the later passes cover bits 9 up in
+// equal digits of at most 9 bits.  Every plan sorts the same keys stably, so
+// all give the same order; the host and k_plan call this with the same inputs.
+__host__ __device__ inline void plan_frame(bool any, uint32_t dmin, uint32_t dmax, uint32_t tmin,
+                                           uint32_t tmax, bool fuse, SortPlan& p) {
+  plan_digits(any, dmin, dmax, tmin, tmax, p);
+  if (!fuse || !any || p.mode != 1) return;
+  const uint32_t da = dmin & ~kEmitDigitMask;
+  int db = bit_width(dmax - da);
+  if (db < kEmitDigitBits) db = kEmitDigitBits;
+  const int tb = bit_width(tmax - tmin);
+  if (db + tb > 32) return;
+  const int rest = db + tb - kEmitDigitBits;
+  const int np = (rest + kMaxDigitBits - 1) / kMaxDigitBits;
+  if (np >= p.passes) return;  // no pass saved
+  const int w = np ? (rest + np - 1) / np : 0;
+  p.dmin = da;
+  p.dbits = db;
+  p.passes = 1 + np;
+  p.first = 1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k == 0) {
+      p.shift[k] = 0;
+      p.width[k] = kEmitDigitBits;
+    } else if (k <= np) {
+      const int s = (k - 1) * w;
+      p.shift[k] = kEmitDigitBits + s;
+      p.width[k] = rest - s < w ? rest - s : w;
+    } else {
+      p.shift[k] = p.width[k] = 0;
     }
   }
 }

@@ -86,7 +86,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
   SortSeg sg;
   uint32_t lb;
   const bool mine = sort_block(segs, nseg, blockIdx.x, sg, lb);
-  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
+  if (pass >= plan->passes || pass < plan->first || !sort_mode_is<K>(plan) ||
+      plan->items != ITEMS)
+    return;
   if (!mine) return;  // surplus block
   const int t = threadIdx.x;
   const K* __restrict__ keys = (pass & 1) ? k1 : k0;
@@ -130,25 +132,17 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
     hist[hist_index(sg, (uint32_t)d, lb)] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
-// Exclusive prefix sums of each (segment, digit) column of the histogram
-// table over the segment's blocks, in place, and the column totals
-// (totals[segment * kDigitBins + digit]).  Grid: segments x kDigitBins.
+// Exclusive prefix sums of col[0, n) in place, by one block of kSortThreads
+// (s: kColChunk + kColChunk / 32 words of LDS, s_w: one per wave); returns
+// the column's total.
 constexpr int kColItems = 8;
-__global__ __launch_bounds__(kSortThreads) void k_sort_colscan(const SortPlan* __restrict__ plan,
-                                                              const SortSeg* __restrict__ segs,
-                                                              int pass, uint32_t* __restrict__ hist,
-                                                              uint32_t* __restrict__ totals) {
-  constexpr int kChunk = kSortThreads * kColItems;
-  __shared__ uint32_t s[kChunk + kChunk / 32];
-  __shared__ uint32_t s_w[kSortThreads / 64];
-  const uint32_t seg = blockIdx.x / kDigitBins, d = blockIdx.x % kDigitBins;
-  const SortSeg sg = segs[seg];
-  if (pass >= plan->passes) return;
-  uint32_t* __restrict__ col = hist + (size_t)sg.block_base * kDigitBins + (size_t)d * sg.nblocks;
+constexpr int kColChunk = kSortThreads * kColItems;
+__device__ inline uint32_t column_scan(uint32_t* __restrict__ col, uint32_t n_all, uint32_t* s,
+                                       uint32_t* s_w) {
   const int t = threadIdx.x;
   uint32_t carry = 0;
-  for (uint32_t b0 = 0; b0 < sg.nblocks; b0 += kChunk) {
-    const uint32_t n = min((uint32_t)kChunk, sg.nblocks - b0);
+  for (uint32_t b0 = 0; b0 < n_all; b0 += kColChunk) {
+    const uint32_t n = min((uint32_t)kColChunk, n_all - b0);
 #pragma unroll
     for (int j = 0; j < kColItems; ++j) {
       const int li = j * kSortThreads + t;
@@ -177,7 +171,24 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_colscan(const SortPlan* _
     carry += tot;
     __syncthreads();  // s is refilled next round
   }
-  if (t == 0) totals[blockIdx.x] = carry;
+  return carry;
+}
+
+// Exclusive prefix sums of each (segment, digit) column of the histogram
+// table over the segment's blocks, in place, and the column totals
+// (totals[segment * kDigitBins + digit]).  Grid: segments x kDigitBins.
+__global__ __launch_bounds__(kSortThreads) void k_sort_colscan(const SortPlan* __restrict__ plan,
+                                                              const SortSeg* __restrict__ segs,
+                                                              int pass, uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s[kColChunk + kColChunk / 32];
+  __shared__ uint32_t s_w[kSortThreads / 64];
+  const uint32_t seg = blockIdx.x / kDigitBins, d = blockIdx.x % kDigitBins;
+  const SortSeg sg = segs[seg];
+  if (pass >= plan->passes || pass < plan->first) return;
+  uint32_t* __restrict__ col = hist + (size_t)sg.block_base * kDigitBins + (size_t)d * sg.nblocks;
+  const uint32_t tot = column_scan(col, sg.nblocks, s, s_w);
+  if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
 template <typename K, int ITEMS>
@@ -199,7 +210,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   uint32_t lb;
   int seg_index = 0;
   const bool mine = sort_block(segs, nseg, blockIdx.x, sg, lb, &seg_index);
-  if (pass >= plan->passes || !sort_mode_is<K>(plan) || plan->items != ITEMS) return;
+  if (pass >= plan->passes || pass < plan->first || !sort_mode_is<K>(plan) ||
+      plan->items != ITEMS)
+    return;
   if (!mine) return;  // surplus block
   const uint32_t m = sg.key_base + sg.count;  // end of this block's segment
   const K* __restrict__ kin = (pass & 1) ? k1 : k0;
@@ -335,14 +348,15 @@ inline size_t sort_part_bytes(uint64_t m_cap, int nseg, int items) {
   return (size_t)(nseg > 0 ? nseg : 1) * kDigitBins * sizeof(uint32_t);
 }
 
-// The passes of one tile size: `slots` pass slots over (k[0], v[0]).
+// The passes of one tile size: pass slots [first, slots) over (k[0], v[0]).
 template <int ITEMS>
 inline LumenStatus sort_passes(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
-                               const SortSeg* segs, int nseg, uint64_t m_cap, int slots, int modes,
-                               uint32_t* hist, uint32_t* part, hipStream_t st) {
+                               const SortSeg* segs, int nseg, uint64_t m_cap, int first,
+                               int slots, int modes, uint32_t* hist, uint32_t* part,
+                               hipStream_t st) {
   const uint32_t nb = sort_grid(m_cap, nseg, ITEMS);
   uint32_t* k32[2] = {reinterpret_cast<uint32_t*>(k[0]), reinterpret_cast<uint32_t*>(k[1])};
-  for (int p = 0; p < slots; ++p) {
+  for (int p = first; p < slots; ++p) {
     if (modes & 1) {
       hipLaunchKernelGGL((k_sort_hist<uint64_t, ITEMS>), dim3(nb), dim3(kSortThreads), 0, st, k[0],
                          k[1], plan, segs, nseg, p, hist);
@@ -372,25 +386,28 @@ inline LumenStatus sort_passes(uint64_t* k[2], uint32_t* v[2], const SortPlan* p
   return LUMEN_SUCCESS;
 }
 
-// Enqueues `slots` pass slots over (k[0], v[0]) with the plan in device
-// memory; the result ends in buffer (plan->passes & 1), as 64-bit keys
+// Enqueues pass slots [first, slots) over (k[0], v[0]) with the plan in
+// device memory (a slot runs when plan->first <= slot < plan->passes; pass p
+// reads buffer p & 1, so a frame whose pass 0 ran in k_emit starts in
+// buffer 1); the result ends in buffer (plan->passes & 1), as 64-bit keys
 // (mode 0) or 32-bit frame keys (mode 1) in the same buffers.  `modes` says
 // which key widths to enqueue (bit 0: 64-bit, bit 1: 32-bit; a graph, which
 // cannot know the frame's mode, takes both).  m_cap bounds the number of
 // keys the plan may name; `items` (12 or 16) is the plan's keys per thread,
 // which the segments' block counts were made with.
 inline LumenStatus radix_sort_planned(uint64_t* k[2], uint32_t* v[2], const SortPlan* plan,
-                                      const SortSeg* segs, int nseg, uint64_t m_cap, int slots,
-                                      int modes, int items, SortScratch sc, hipStream_t st) {
-  if (m_cap == 0 || slots == 0) return LUMEN_SUCCESS;
+                                      const SortSeg* segs, int nseg, uint64_t m_cap, int first,
+                                      int slots, int modes, int items, SortScratch sc,
+                                      hipStream_t st) {
+  if (m_cap == 0 || slots <= first) return LUMEN_SUCCESS;
   if (items != 12 && items != 16) return fail(LUMEN_ERROR_INVALID_ARGUMENT, "sort: %d keys per thread", items);
   LUMEN_TRY(ensure(*sc.hist, sort_hist_len(m_cap, nseg, items) * sizeof(uint32_t)));
   LUMEN_TRY(ensure(*sc.hist_part, sort_part_bytes(m_cap, nseg, items)));
   uint32_t* hist = sc.hist->as<uint32_t>();
   uint32_t* part = sc.hist_part->as<uint32_t>();
   if (items == 12)
-    return sort_passes<12>(k, v, plan, segs, nseg, m_cap, slots, modes, hist, part, st);
-  return sort_passes<16>(k, v, plan, segs, nseg, m_cap, slots, modes, hist, part, st);
+    return sort_passes<12>(k, v, plan, segs, nseg, m_cap, first, slots, modes, hist, part, st);
+  return sort_passes<16>(k, v, plan, segs, nseg, m_cap, first, slots, modes, hist, part, st);
 }
 
 }  // namespace lumen

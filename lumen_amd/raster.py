@@ -70,6 +70,7 @@ class RenderOutput:
     total_splat_instances: int
     sort_passes: int = 0
     key_bytes: int = 8
+    emit_passes: int = 0  # of sort_passes, run while the keys were emitted
 
     def get_pixel_rgb(self, x: int, y: int):
         if 0 <= x < self.width and 0 <= y < self.height:
@@ -140,7 +141,7 @@ class Rasterizer:
                       int(stats.tiles_x * stats.tiles_y), W * H)
         return RenderOutput(W, H, out_rgb, out_depth, int(stats.visible_splats),
                             int(stats.total_splat_instances), int(stats.sort_passes),
-                            int(stats.key_bytes))
+                            int(stats.key_bytes), int(stats.emit_passes))
 
     def stage(self, name: str) -> np.ndarray:
         """Host copy of a stage buffer of the last render (see lumen_ex.h)."""

@@ -52,7 +52,7 @@ def test_version(L):
     assert L.lumen_get_version() == b"0.1.0"
     assert (L.lumen_get_version_major(), L.lumen_get_version_minor(),
             L.lumen_get_version_patch()) == (0, 1, 0)
-    assert L.lumen_ex_abi_version() == 1
+    assert L.lumen_ex_abi_version() == 2
 
 
 def test_config(L):

@@ -63,6 +63,32 @@ def compare(rasterizer, oracle, scene, camera, config=None, exact_images=True):
                                       ref["t_final"])
         np.testing.assert_array_equal(rasterizer.stage("n_eval").reshape(ref["n_eval"].shape),
                                       ref["n_eval"])
+    # the same frame without the kept unsorted stages: single-view frames of
+    # at least 2048 tiles then run the sort's first pass inside the key
+    # emission (keys written in that pass's order), the others emit in the
+    # reference's order; the sorted keys, values, ranges and pixels must not
+    # change either way
+    out2 = rasterizer.render(scene, camera, config, save_pixels=True)
+    torch.cuda.synchronize()
+    assert out.emit_passes == 0  # kept stages: the reference's emission order
+    tiles = math.ceil(camera.config.width / config.tile_width) * \
+        math.ceil(camera.config.height / config.tile_height)
+    if tiles < 2048 or M == 0:
+        assert out2.emit_passes == 0
+    say(f"[parity]   unkept frame: {out2.sort_passes} passes, {out2.emit_passes} in emit")
+    assert out2.sort_passes == out.sort_passes
+    if M:
+        np.testing.assert_array_equal(rasterizer.stage("keys"), ref["keys"])
+        np.testing.assert_array_equal(rasterizer.stage("vals"), ref["vals"])
+    np.testing.assert_array_equal(rasterizer.stage("ranges"), ref["ranges"])
+    rgb2 = out2.rgb.cpu().numpy()
+    if exact_images:
+        np.testing.assert_array_equal(rgb2, ref["rgb"])
+        np.testing.assert_array_equal(out2.depth.cpu().numpy(), ref["depth"])
+        np.testing.assert_array_equal(rasterizer.stage("n_eval").reshape(ref["n_eval"].shape),
+                                      ref["n_eval"])
+    else:
+        np.testing.assert_array_equal(rgb2, rgb)
     return ref, out
 
 
@@ -76,6 +102,8 @@ def test_random100k_1280x720(rasterizer, oracle):
     sc, cam = workload("random100k_1280x720")
     ref, out = compare(rasterizer, oracle, sc, cam)
     assert ref["instances"] > 400_000
+    # 3600 tiles: the unkept frame's first pass runs in the key emission
+    assert rasterizer.render(sc, cam, RenderConfig()).emit_passes == 1
 
 
 def test_capacity_growth_and_reuse(oracle):
@@ -216,6 +244,9 @@ def test_random1m_1920x1080_full_parity(rasterizer, oracle):
     sc, cam = workload("random1m_1920x1080")
     ref, out = compare(rasterizer, oracle, sc, cam)
     say(f"[parity] 1M: V={ref['visible']} M={ref['instances']} E={ref['evaluations']}")
+    # 27-bit frame keys: the key emission runs the first of the 3 passes
+    o = rasterizer.render(sc, cam, RenderConfig())
+    assert (o.sort_passes, o.emit_passes) == (3, 1)
 
 
 @pytest.mark.parametrize("eye", [(1.3, 3.9, 2.8), (1.3, 0.7, 4.8)])
