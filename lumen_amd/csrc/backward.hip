@@ -178,19 +178,17 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const uint32_t min_tx = mt.y & 0xFFFFu, min_ty = mt.y >> 16;
       const uint32_t wtiles = (mt.z & 0xFFFFu) - min_tx + 1u;
       s_dst[j] = mt.x + ((uint32_t)ty - min_ty) * wtiles + ((uint32_t)tx - min_tx);
-      const float4 a = rec[3u * vi + 0u];
-      const float4 b = rec[3u * vi + 1u];
-      const float4 c = rec[3u * vi + 2u];
-      const float lim = power_limit(b.y);
-      float hx = c.z, hy = c.w;
-      limit_box(lim, hx, hy);
+      SplatRecord r;
+      r.q0 = rec[4u * vi + 0u];
+      r.q1 = rec[4u * vi + 1u];
+      r.q2 = rec[4u * vi + 2u];
+      r.q3 = rec[4u * vi + 3u];
       // the conic halved as the forward stages it, so the power below is the
       // forward's expression itself
-      s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
-      s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
-      s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-      s_mask[j] = (uint8_t)block_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                      tx * v.tile_w, ty * v.tile_h, qw, qh, nwx, nwaves);
+      s0[j] = r.q0;  // x, y, i00/2, i01
+      s1[j] = r.q1;  // i11/2, opacity, power limit
+      s2[j] = r.q2;  // r, g, b, depth
+      s_mask[j] = (uint8_t)record_mask(r, tx * v.tile_w, ty * v.tile_h, qw, qh, nwx, nwaves);
     }
     __syncthreads();
     // this wave's list: splats up to its pixels' last evaluated one whose box

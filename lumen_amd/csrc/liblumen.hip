@@ -572,9 +572,11 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
       float i00, i01, i11, hx, hy;
       inverse2d(c00, c01, c11, i00, i01, i11);
       splat_box(i00, i01, i11, hx, hy);
-      rec[3u * vi + 0u] = make_float4(sx, sy, i00, 2.0f * i01);
-      rec[3u * vi + 1u] = make_float4(i11, op, cr, cg);
-      rec[3u * vi + 2u] = make_float4(cb, depth, hx, hy);
+      const SplatRecord sr = make_record(sx, sy, i00, i01, i11, op, cr, cg, cb, depth, hx, hy);
+      rec[4u * vi + 0u] = sr.q0;
+      rec[4u * vi + 1u] = sr.q1;
+      rec[4u * vi + 2u] = sr.q2;
+      rec[4u * vi + 3u] = sr.q3;
       vis2orig[vi] = (int32_t)i;
       radius[vi] = radius_of(c00, c01, c11);
       vmeta[vi] = make_uint4(ko, rmin, rmax, 0u);
@@ -944,21 +946,19 @@ __global__ __launch_bounds__(256) void k_composite(
       const int n = min(BATCH, rg.y - base);
       for (int j = t; j < n; j += nthreads) {
         const uint32_t vi = vals[base + j];
-        const float4 a = rec[3u * vi + 0u];
-        const float4 b = rec[3u * vi + 1u];
-        const float4 c = rec[3u * vi + 2u];
-        const float lim = power_limit(b.y);
-        float hx = c.z, hy = c.w;
-        limit_box(lim, hx, hy);
-        // the power's factor 0.5 goes into the conic.  Halving commutes with
-        // each rounding of 0.5 (i00 dx dx + 2 i01 dx dy + i11 dy dy) while the
-        // terms are normal floats, so the power keeps its bits; a power small
-        // enough for that to fail (< 2^-125) gives exp(-power) = 1 either way
-        s0[j] = make_float4(a.x, a.y, 0.5f * a.z, 0.5f * a.w);  // x, y, i00/2, i01
-        s1[j] = make_float4(0.5f * b.x, b.y, lim, 0.0f);      // i11/2, opacity, power limit
-        s2[j] = make_float4(b.z, b.w, c.x, c.y);  // r, g, b, depth
-        s_mask[j] = (uint8_t)strip_mask(a.x, a.y, hx, hy, a.z, 0.5f * a.w, b.x, lim,
-                                        tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, nwaves);
+        SplatRecord r;
+        r.q0 = rec[4u * vi + 0u];
+        r.q1 = rec[4u * vi + 1u];
+        r.q2 = rec[4u * vi + 2u];
+        r.q3 = rec[4u * vi + 3u];
+        // the conic comes halved (the power's factor 0.5; a power small
+        // enough for the halving to lose bits, < 2^-125, gives exp(-power) =
+        // 1 either way)
+        s0[j] = r.q0;  // x, y, i00/2, i01
+        s1[j] = r.q1;  // i11/2, opacity, power limit (the cut is not used here)
+        s2[j] = r.q2;  // r, g, b, depth
+        s_mask[j] = (uint8_t)record_mask(r, tx * v.tile_w, ty * v.tile_h, v.tile_w, rows, 1,
+                                         nwaves);
       }
       __syncthreads();
       // this wave's list of the batch's splats, in order
@@ -1032,6 +1032,18 @@ __global__ __launch_bounds__(256) void k_composite(
     if (t_final) t_final[pix] = T[p];
     if (n_eval) n_eval[pix] = ne[p];
   }
+}
+
+// The records in the 48-byte layout lumen_ex_copy_stage documents (cx cy i00
+// 2 i01 | i11 op r g | b depth 0 0): the reference's projected splat, from
+// the 64-byte compositor records (doubling the halved conic is exact).
+__global__ void k_records_out(const float4* __restrict__ rec, uint32_t v, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v) return;
+  const float4 q0 = rec[4u * i + 0u], q1 = rec[4u * i + 1u], q2 = rec[4u * i + 2u];
+  out[3u * i + 0u] = make_float4(q0.x, q0.y, 2.0f * q0.z, 2.0f * q0.w);
+  out[3u * i + 1u] = make_float4(2.0f * q1.x, q1.y, q2.x, q2.y);
+  out[3u * i + 2u] = make_float4(q2.z, q2.w, 0.0f, 0.0f);
 }
 
 __global__ void k_minmax_u64(const uint64_t* __restrict__ keys, uint64_t n,
@@ -1275,7 +1287,7 @@ static LumenStatus ensure_frame_buffers(LumenExContext* c, uint64_t N, const Fra
 template <class Need>
 static LumenStatus capacity_buffers(LumenExContext* c, Need&& need, uint64_t v_cap,
                                     uint64_t m_cap) {
-  LUMEN_TRY(need(c->rec, (v_cap ? v_cap : 1) * 48));
+  LUMEN_TRY(need(c->rec, (v_cap ? v_cap : 1) * sizeof(SplatRecord)));
   LUMEN_TRY(need(c->vis2orig, (v_cap ? v_cap : 1) * 4));
   LUMEN_TRY(need(c->radius, (v_cap ? v_cap : 1) * 4));
   LUMEN_TRY(need(c->vmeta, (v_cap ? v_cap : 1) * 16));
@@ -1442,7 +1454,7 @@ static LumenStatus enqueue_rest(LumenExContext* c, const SceneDev& sd, const Fra
 // frame has sized them).
 static void current_capacity(const LumenExContext* c, uint64_t& v_cap, uint64_t& m_cap) {
   auto cap = [](const DevBuf& b, uint64_t unit) -> uint64_t { return b.ptr ? b.cap / unit : 0; };
-  v_cap = cap(c->rec, 48);
+  v_cap = cap(c->rec, sizeof(SplatRecord));
   v_cap = std::min(v_cap, cap(c->vis2orig, 4));
   v_cap = std::min(v_cap, cap(c->radius, 4));
   v_cap = std::min(v_cap, cap(c->vmeta, 16));
@@ -2101,7 +2113,23 @@ LumenStatus lumen_ex_copy_stage(LumenExContext* c, int stage, void* dst, size_t 
   size_t want = 0;
   switch (stage) {
     case LUMEN_EX_STAGE_VIS_TO_ORIG: src = c->vis2orig.ptr; want = c->V * 4; break;
-    case LUMEN_EX_STAGE_RECORDS: src = c->rec.ptr; want = c->V * 48; break;
+    case LUMEN_EX_STAGE_RECORDS: {  // converted to the documented 48-byte layout
+      want = c->V * 48;
+      if (bytes != want) break;
+      if (want == 0) return LUMEN_SUCCESS;
+      DevBuf tmp;
+      LUMEN_HIP(hipSetDevice(c->device));
+      LUMEN_TRY(ensure(tmp, want));
+      hipLaunchKernelGGL(k_records_out, dim3(div_up(c->V, 256)), dim3(256), 0, nullptr,
+                         c->rec.as<float4>(), (uint32_t)c->V, tmp.as<float4>());
+      const hipError_t e1 = hipGetLastError();
+      const hipError_t e2 = e1 == hipSuccess ? hipDeviceSynchronize() : e1;
+      const hipError_t e3 = e2 == hipSuccess ? hipMemcpy(dst, tmp.ptr, want, hipMemcpyDeviceToHost) : e2;
+      release(tmp);
+      if (e3 != hipSuccess)
+        return fail(LUMEN_EX_ERROR_DEVICE, "lumen_ex_copy_stage: %s", hipGetErrorString(e3));
+      return LUMEN_SUCCESS;
+    }
     case LUMEN_EX_STAGE_RADIUS: src = c->radius.ptr; want = c->V * 4; break;
     case LUMEN_EX_STAGE_KEYS_UNSORTED:
       if (!c->kept && c->M) return fail(LUMEN_ERROR_UNINITIALIZED, "stage not kept (LUMEN_EX_KEEP_STAGES)");

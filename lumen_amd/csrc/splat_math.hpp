@@ -415,6 +415,91 @@ __device__ __forceinline__ uint32_t strip_mask(float sx, float sy, float hx, flo
   return m;
 }
 
+// ---- the splat record -------------------------------------------------------
+// k_emit writes one 64-byte record per visible splat; the compositors stage
+// it per tile.  Everything that depends on the splat alone is computed there
+// once, not per (splat, tile):
+//   q0 = (x, y, i00 / 2, i01)        centre and the halved conic (the power
+//   q1 = (i11 / 2, opacity, lim, cut)  0.5 (i00 dx dx + 2 i01 dx dy + i11 dy dy)
+//   q2 = (r, g, b, depth)              with its factor 0.5 folded in: halving
+//   q3 = (hx, hy, 1 / i00, 1 / i11)    commutes with each rounding of normal
+//                                      floats, so the power keeps its bits)
+// lim is power_limit(opacity), (hx, hy) the splat's box at that limit
+// (limit_box), cut = 1.02 lim + 1e-3 the strip test's bound on the power's
+// minimum over a wave's pixels, or +inf where that test does not apply
+// (conics with condition number above 1e4, singular or indefinite ones, NaN
+// centres or limits), and 1 / i00, 1 / i11 the stationary-point factors of
+// rect_min_power.
+struct SplatRecord {
+  float4 q0, q1, q2, q3;
+};
+
+__device__ __forceinline__ SplatRecord make_record(float sx, float sy, float i00, float i01,
+                                                   float i11, float op, float r, float g,
+                                                   float b, float depth, float hx, float hy) {
+  const float lim = power_limit(op);
+  limit_box(lim, hx, hy);
+  const float det = i00 * i11 - i01 * i01;
+  const float tr = i00 + i11;
+  const bool exact = i00 > 0.0f && i11 > 0.0f && det > 0.0f && tr * tr <= 1.0e4f * det &&
+                     sx == sx && sy == sy && lim == lim;
+  SplatRecord rec;
+  rec.q0 = make_float4(sx, sy, 0.5f * i00, i01);
+  rec.q1 = make_float4(0.5f * i11, op, lim, exact ? 1.02f * lim + 1e-3f : __builtin_inff());
+  rec.q2 = make_float4(r, g, b, depth);
+  rec.q3 = make_float4(hx, hy, 1.0f / i00, 1.0f / i11);
+  return rec;
+}
+
+// rect_min_power with the record's reciprocals (i00 = 2 q0.z, i11 = 2 q1.x).
+__device__ __forceinline__ float rect_min_power_rec(float sx, float sy, float i00, float i01,
+                                                    float i11, float r00, float r11, float x0,
+                                                    float x1, float y0, float y1) {
+  if (sx >= x0 && sx <= x1 && sy >= y0 && sy <= y1) return 0.0f;
+  auto q = [&](float dx, float dy) {
+    return 0.5f * (i00 * dx * dx + 2.0f * i01 * dx * dy + i11 * dy * dy);
+  };
+  float m = __builtin_inff();
+  const float ys[2] = {y0, y1}, xs[2] = {x0, x1};
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dy = ys[e] - sy;  // edge y = ys[e]: least at dx = -i01 dy / i00
+    const float dx = fminf(fmaxf(-i01 * dy * r00, x0 - sx), x1 - sx);
+    m = fminf(m, q(dx, dy));
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dx = xs[e] - sx;  // edge x = xs[e]: least at dy = -i01 dx / i11
+    const float dy = fminf(fmaxf(-i01 * dx * r11, y0 - sy), y1 - sy);
+    m = fminf(m, q(dx, dy));
+  }
+  return m;
+}
+
+// The waves whose pixel centres a record's splat can reach at its power
+// limit: bit w set unless the record's box misses wave w's block of qw x qh
+// pixels (column w % nwx, row w / nwx of the tile) or the power's minimum
+// over the block's pixel centres exceeds the record's cut.  The same answer
+// strip_mask / block_mask give from the splat's raw quantities (full-width
+// strips are blocks of qw = tile width).
+__device__ __forceinline__ uint32_t record_mask(const SplatRecord& r, int tile_x0, int tile_y0,
+                                                int qw, int qh, int nwx, int nwaves) {
+  const float sx = r.q0.x, sy = r.q0.y, hx = r.q3.x, hy = r.q3.y, cut = r.q1.w;
+  const float i00 = 2.0f * r.q0.z, i01 = r.q0.w, i11 = 2.0f * r.q1.x;
+  uint32_t m = 0u;
+  for (int w = 0; w < nwaves; ++w) {
+    const int bx = tile_x0 + (w % nwx) * qw, by = tile_y0 + (w / nwx) * qh;
+    const float x0 = (float)bx + 0.5f, x1 = (float)(bx + qw) - 0.5f;
+    const float y0 = (float)by + 0.5f, y1 = (float)(by + qh) - 0.5f;
+    if (sx + hx < x0 || sx - hx > x1 || sy + hy < y0 || sy - hy > y1) continue;
+    if (cut < __builtin_inff() &&
+        rect_min_power_rec(sx, sy, i00, i01, i11, r.q3.z, r.q3.w, x0, x1, y0, y1) > cut)
+      continue;
+    m |= 1u << w;
+  }
+  return m;
+}
+
 // strip_mask for waves that cover qw x qh blocks of the tile (wave w: block
 // column w % nwx, row w / nwx): bit w set unless the splat's box misses the
 // block's pixel centres or, for conics with condition number up to 1e4, its
