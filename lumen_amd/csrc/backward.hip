@@ -68,7 +68,9 @@ __global__ void k_chunk_seg(const SortPlan* __restrict__ plan, const uint4* __re
   }
 }
 
-template <int BATCH, int PIX>
+// DEPTH: a depth gradient is given (without it the depth channel's terms,
+// one of the ten per-splat sums included, are left out: they are zero).
+template <int BATCH, int PIX, bool DEPTH>
 __global__ __launch_bounds__(256) void k_composite_bwd(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       gC0[p] = g_rgb[3 * pix + 0];
       gC1[p] = g_rgb[3 * pix + 1];
       gC2[p] = g_rgb[3 * pix + 2];
-      gD[p] = (g_depth && v.compute_depth) ? g_depth[pix] : 0.0f;
+      gD[p] = (DEPTH && v.compute_depth) ? g_depth[pix] : 0.0f;
     }
     if (white) Q[p] = (gC0[p] + gC1[p] + gC2[p]) * T[p];
     lane_last = max(lane_last, last[p]);
@@ -241,7 +243,8 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
             const float w = alpha * Tj;
             // dL/dalpha = sum_k g_k (c_k Tj - S_k / (1 - alpha)), S_k the
             // channel blended behind; sum_k g_k S_k is carried as Q
-            const float gc = gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z + gD[p] * c.w;
+            const float gc = DEPTH ? gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z + gD[p] * c.w
+                                   : gC0[p] * c.x + gC1[p] * c.y + gC2[p] * c.z;
             const float dal = gc * Tj - Q[p] * rom;
             Q[p] += gc * w;
             T[p] = Tj;
@@ -259,19 +262,31 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         const float m0 = dpm * dx, m1 = dpm * dy;
         const float t[kGradVis] = {m0,          m1,          m0 * dx,     m0 * dy,
                                    m1 * dy,     g5,          gC0[p] * wm, gC1[p] * wm,
-                                   gC2[p] * wm, gD[p] * wm};
+                                   gC2[p] * wm, DEPTH ? gD[p] * wm : 0.0f};
 #pragma unroll
         for (int k = 0; k < kGradVis; ++k) gv[k] = p == 0 ? t[k] : gv[k] + t[k];
       }
       if (__ballot(contrib)) {  // wave-uniform; the whole wave is active here
-        float sa, sb, sc;
-        wave_sum10(gv, sa, sb, sc);
-        if ((lane & 15) == 15) {
-          const int r = lane >> 4;
-          float* dst = &s_part[wv][j * kGradVis];
-          dst[sum10_slot(r)] = sa;
-          dst[4 + sum10_slot(r)] = sb;
-          if ((r & 1) == 0) dst[8 + (r >> 1)] = sc;
+        float* dst = &s_part[wv][j * kGradVis];
+        if (DEPTH) {
+          float sa, sb, sc;
+          wave_sum10(gv, sa, sb, sc);
+          if ((lane & 15) == 15) {
+            const int r = lane >> 4;
+            dst[sum10_slot(r)] = sa;
+            dst[4 + sum10_slot(r)] = sb;
+            if ((r & 1) == 0) dst[8 + (r >> 1)] = sc;
+          }
+        } else {  // nine sums (the depth slot stays zero)
+          float sa, sb;
+          wave_sum8(gv, sa, sb);
+          const float sc = wave_sum_lane63(gv[8]);
+          if ((lane & 15) == 15) {
+            const int r = lane >> 4;
+            dst[sum10_slot(r)] = sa;
+            dst[4 + sum10_slot(r)] = sb;
+            if (r == 3) dst[8] = sc;
+          }
         }
       }
     }
@@ -816,14 +831,15 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   // state of two pixels costs resident waves)
   const dim3 grid((uint32_t)c->T), block(threads);
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, false, st));
-  if (batch == 64)
-    hipLaunchKernelGGL((k_composite_bwd<64, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
-                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part,
-                       c->tile_order.as<uint32_t>());
-  else
-    hipLaunchKernelGGL((k_composite_bwd<128, 1>), grid, block, 0, st, ranges, v0, v1, plan, rec,
-                       views, nv, spans, vmeta, tf, ne, grad_rgb, grad_depth, part,
-                       c->tile_order.as<uint32_t>());
+  // the depth channel's terms only where a depth gradient is given and a
+  // view computes depth
+  bool depth = false;
+  for (const ViewParams& v : c->host_views) depth = depth || v.compute_depth;
+  depth = depth && grad_depth != nullptr;
+  auto kb = batch == 64 ? (depth ? k_composite_bwd<64, 1, true> : k_composite_bwd<64, 1, false>)
+                        : (depth ? k_composite_bwd<128, 1, true> : k_composite_bwd<128, 1, false>);
+  hipLaunchKernelGGL(kb, grid, block, 0, st, ranges, v0, v1, plan, rec, views, nv, spans, vmeta,
+                     tf, ne, grad_rgb, grad_depth, part, c->tile_order.as<uint32_t>());
   LUMEN_LAUNCHED();
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_COMPOSITE, true, st));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_BWD_PREPROCESS, false, st));

@@ -165,6 +165,12 @@ __device__ __forceinline__ void wave_sum10(const float v[10], float& a, float& b
   c = row_sum_lane15(swap16_add(r, r));
 }
 
+// The first eight values of wave_sum10, in the same lanes of a and b.
+__device__ __forceinline__ void wave_sum8(const float v[8], float& a, float& b) {
+  a = row_sum_lane15(swap16_add(swap32_add(v[0], v[1]), swap32_add(v[2], v[3])));
+  b = row_sum_lane15(swap16_add(swap32_add(v[4], v[5]), swap32_add(v[6], v[7])));
+}
+
 __device__ __forceinline__ uint32_t wave_min_u32_all(uint32_t x) {
   x = min(x, dpp_u32<0x111, 0xF>(x, 0xFFFFFFFFu));
   x = min(x, dpp_u32<0x112, 0xF>(x, 0xFFFFFFFFu));
