@@ -452,17 +452,25 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
   constexpr int NW = BLOCK / 64;
   // digits per thread for the table row (512 digits over the block's threads)
   constexpr int DPT = kDigitBins / BLOCK > 0 ? kDigitBins / BLOCK : 1;
-  // per thread of the block: (key start - block base, rect min, rect max,
-  // depth key) and the visible index
-  __shared__ uint4 s_meta[BLOCK];
-  __shared__ uint32_t s_vi[BLOCK];
-  // first pass in k_emit: each thread's output start, per-wave digit sums
-  // (then their exclusive prefixes over the waves) and the block's start of
-  // each digit in the view's keys
-  __shared__ uint32_t s_dst[FUSABLE ? BLOCK : 1];
-  __shared__ uint32_t s_wh[FUSABLE ? NW : 1][kDigitBins];
-  __shared__ uint32_t s_delta[kDigitBins];
+  // LDS, in one block: first (single view) each wave's SH staging, 16
+  // Gaussians of 12 quads in rows padded to kShPad quads; then per thread
+  // of the block (key start - block base, rect min, rect max, depth key) and
+  // the visible index, and for the first sort pass each thread's output
+  // start, per-wave digit sums (then their exclusive prefixes over the
+  // waves) and the block's start of each digit in the view's keys
+  constexpr int kShPad = 13;
+  constexpr size_t kShBytes = MULTI ? 0 : (size_t)NW * 16 * kShPad * sizeof(float4);
+  constexpr size_t kMetaBytes = (size_t)BLOCK * sizeof(uint4) + (size_t)BLOCK * 4 +
+                                (FUSABLE ? (size_t)BLOCK * 4 + (size_t)NW * kDigitBins * 4 : 0) +
+                                (size_t)kDigitBins * 4;
+  __shared__ __align__(16) uint8_t s_raw[kShBytes > kMetaBytes ? kShBytes : kMetaBytes];
   __shared__ uint32_t s_w[NW];
+  float4* const s_sh = reinterpret_cast<float4*>(s_raw);
+  uint4* const s_meta = reinterpret_cast<uint4*>(s_raw);
+  uint32_t* const s_vi = reinterpret_cast<uint32_t*>(s_meta + BLOCK);
+  uint32_t* const s_delta = s_vi + BLOCK;
+  uint32_t* const s_dst = s_delta + kDigitBins;
+  uint32_t (*const s_wh)[kDigitBins] = reinterpret_cast<uint32_t (*)[kDigitBins]>(s_dst + BLOCK);
   if (plan->overflow) return;  // k_plan: the frame exceeds the capacities
   const bool fused = FUSABLE && plan->first == 1;  // only ever with etab (k_plan's fuse)
   const int t = threadIdx.x;
@@ -520,7 +528,6 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
         }
       }
       vkb = (uint32_t)(offsets[row] >> 32);  // the view's first key
-      for (int e = t; e < NW * kDigitBins; e += BLOCK) (&s_wh[0][0])[e] = 0u;
     }
     uint32_t vis = 0, ko = base, vi = 0, cnt = 0;
     if (valid) {
@@ -532,13 +539,67 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
       cnt = (uint32_t)(off1 >> 32) - ko;
     }
     uint32_t rmin = 0, rmax = 0, dk = 0;
+    if (!MULTI) {
+      // The wave's SH coefficients, loaded together 16 Gaussians at a time:
+      // their 3 KB are contiguous, so each load instruction reads 1 KB of
+      // consecutive addresses (a lane loading its own 192 B at a 192 B
+      // stride meets a different cache line in every lane), and pass through
+      // LDS (rows padded to 13 quads, so the 16 readers' quads fall in
+      // distinct banks) to the lanes that own them.  The next group's loads
+      // are issued before this group's exchange.  Only visible splats'
+      // quads are loaded.
+      const uint64_t visb = __ballot(vis != 0u);
+      const int nq = (3 * v.sh_coeffs + 3) / 4;  // quads of coefficients the degree uses
+      const uint32_t wfirst = first + (uint32_t)(w * 64);
+      const float4* __restrict__ sh4 = reinterpret_cast<const float4*>(s.sh);
+      float4* __restrict__ shw = s_sh + w * 16 * kShPad;
+      float4 r[2][3];
+      auto fetch = [&](int ph, float4 (&dst)[3]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int idx = k * 64 + lane, gl = idx / 12, q = idx - gl * 12;
+          dst[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (q < nq && ((visb >> (16 * ph + gl)) & 1ull))
+            dst[k] = sh4[(size_t)(wfirst + 16u * ph + gl) * 12u + q];
+        }
+      };
+      fetch(0, r[0]);
+#pragma unroll
+      for (int ph = 0; ph < 4; ++ph) {
+        if (ph < 3) fetch(ph + 1, r[(ph + 1) & 1]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int idx = k * 64 + lane, gl = idx / 12, q = idx - gl * 12;
+          shw[gl * kShPad + q] = r[ph & 1][k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if ((lane >> 4) == ph && vis) {
+#pragma unroll
+          for (int q = 0; q < 12; ++q) {
+            const float4 x = q < nq ? shw[(lane & 15) * kShPad + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            c[4 * q + 0] = x.x;
+            c[4 * q + 1] = x.y;
+            c[4 * q + 2] = x.z;
+            c[4 * q + 3] = x.w;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      __syncthreads();  // the staging's LDS is the tables' below
+    }
+    if (fused)
+      for (int e = t; e < NW * kDigitBins; e += BLOCK) (&s_wh[0][0])[e] = 0u;
     if (vis) {
       const GeoCache gc = geo[row + i];
       const float sx = gc.a.x, sy = gc.a.y, depth = gc.a.z;
       const float c00 = gc.a.w, c01 = gc.b.x, c11 = gc.b.y;
       rmin = __float_as_uint(gc.b.z);
       rmax = __float_as_uint(gc.b.w);
-      if (!MULTI || !have_sh) {
+      if (MULTI && !have_sh) {
         const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
         const int nfloat = 3 * v.sh_coeffs;
 #pragma unroll
