@@ -511,6 +511,14 @@ struct GradPlanes {
 // one thread: no atomics, and the same result on every run for any number
 // of views.  The view's visible index comes from the scanned per-(view,
 // Gaussian) offsets (visible iff the next offset's visible count is larger).
+//
+// SINGLE (one view): a wave's SH coefficients are read, and their gradients
+// written, 16 Gaussians at a time with coalesced accesses (their 3 KB are
+// contiguous), passing through LDS to and from the lanes that own them; a
+// lane reading or writing its own 192 B at a 192 B stride meets a different
+// cache line in every lane.
+constexpr int kBwdShPad = 13;  // LDS rows of 13 quads: the 16 owners hit distinct banks
+template <bool SINGLE>
 __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const ViewParams* __restrict__ views,
                                                         int nviews,
@@ -519,10 +527,20 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
                                                         const SortPlan* __restrict__ plan,
                                                         const float* __restrict__ grad_vis,
                                                         GradPlanes out, bool assign) {
+  __shared__ float4 s_sh[SINGLE ? 4 : 1][SINGLE ? 16 * kBwdShPad : 1];
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= s.n) return;
-  if (plan->overflow) {  // the frame was not rendered: no gradient
-    if (assign) {
+  const bool valid = i < s.n;
+  if (!SINGLE && !valid) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t wfirst = blockIdx.x * 256u + (threadIdx.x & ~63u);
+  float4* __restrict__ shw = &s_sh[SINGLE ? threadIdx.x >> 6 : 0][0];
+  auto wave_sync = []() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  if (plan->overflow) {  // the frame was not rendered: no gradient (uniform)
+    if (valid && assign) {
       for (int k = 0; k < 11; ++k) out.p[k][i] = 0.0f;
       float4* z4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
 #pragma unroll
@@ -536,14 +554,56 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
     else out.p[plane][i] += g;
   };
   const uint64_t nv = (uint64_t)nviews * s.n;
-  const float px = s.px[i], py = s.py[i], pz = s.pz[i];
+  const uint32_t ic = valid ? i : 0u;  // (lanes past the scene read Gaussian 0, unused)
+  const float px = s.px[ic], py = s.py[ic], pz = s.pz[ic];
   const int ncoef = views[0].sh_coeffs;  // the same in every view of a frame
   const int nfloat = 3 * ncoef;
+  const int nq = (nfloat + 3) / 4;  // quads of coefficients the degree uses
   bool loaded = false;
   float c[48];     // SH coefficients (loaded at the first visible view)
   float gsh[48];   // dL/dsh
 #pragma unroll
   for (int e = 0; e < 48; ++e) gsh[e] = 0.0f;
+  if (SINGLE) {  // the coefficients of the wave's splats visible in the view
+    const bool vis0 = valid && (uint32_t)(i + 1 < nv ? offsets[i + 1] : *totals) !=
+                                   (uint32_t)offsets[i];
+    const uint64_t visb = __ballot(vis0);
+    const float4* __restrict__ sh4 = reinterpret_cast<const float4*>(s.sh);
+#pragma unroll
+    for (int e = 0; e < 48; ++e) c[e] = 0.0f;
+    float4 r[2][3];
+    auto fetch = [&](int ph, float4 (&dst)[3]) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int idx = k * 64 + lane, gl = idx / 12, q = idx - gl * 12;
+        dst[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (q < nq && ((visb >> (16 * ph + gl)) & 1ull))
+          dst[k] = sh4[(size_t)(wfirst + 16u * ph + gl) * 12u + q];
+      }
+    };
+    fetch(0, r[0]);
+#pragma unroll
+    for (int ph = 0; ph < 4; ++ph) {
+      if (ph < 3) fetch(ph + 1, r[(ph + 1) & 1]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int idx = k * 64 + lane, gl = idx / 12, q = idx - gl * 12;
+        shw[gl * kBwdShPad + q] = r[ph & 1][k];
+      }
+      wave_sync();
+      if ((lane >> 4) == ph && vis0) {
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+          const float4 x = q < nq ? shw[(lane & 15) * kBwdShPad + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+          c[4 * q + 0] = x.x;
+          c[4 * q + 1] = x.y;
+          c[4 * q + 2] = x.z;
+          c[4 * q + 3] = x.w;
+        }
+      }
+      wave_sync();
+    }
+  }
   float gpos[3] = {0.0f, 0.0f, 0.0f};
   float gop = 0.0f, vs = 0.0f;
   int32_t seen = 0;
@@ -557,7 +617,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) F[rr][cc] = 0.0f;
 
-  for (int vw = 0; vw < nviews; ++vw) {
+  for (int vw = 0; vw < (valid ? nviews : 0); ++vw) {
     const uint64_t idx = (uint64_t)vw * s.n + i;
     const uint64_t off0 = offsets[idx];
     const uint64_t off1 = idx + 1 < nv ? offsets[idx + 1] : *totals;
@@ -566,14 +626,16 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
     const ViewParams& v = views[vw];
     if (!loaded) {
       loaded = true;
-      const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
+      if (!SINGLE) {
+        const float4* sh4 = reinterpret_cast<const float4*>(s.sh) + (size_t)i * 12u;
 #pragma unroll
-      for (int qd = 0; qd < 12; ++qd) {
-        const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
-        c[4 * qd + 0] = x.x;
-        c[4 * qd + 1] = x.y;
-        c[4 * qd + 2] = x.z;
-        c[4 * qd + 3] = x.w;
+        for (int qd = 0; qd < 12; ++qd) {
+          const float4 x = (4 * qd < nfloat) ? sh4[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+          c[4 * qd + 0] = x.x;
+          c[4 * qd + 1] = x.y;
+          c[4 * qd + 2] = x.z;
+          c[4 * qd + 3] = x.w;
+        }
       }
       // world covariance Sigma = M M^T, M = R(q) diag(s), as the forward
       const Quat qraw{s.rw[i], s.rx[i], s.ry[i], s.rz[i]};
@@ -692,12 +754,51 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
     gpc[1] += gj12 * (-fy * itz2);
     for (int a = 0; a < 3; ++a) gpos[a] += W[a] * gpc[0] + W[3 + a] * gpc[1] + W[6 + a] * gpc[2];
   }
+  if (SINGLE) {
+    // dL/dsh of the wave's splats (assigning: every element of every splat,
+    // zero past the used coefficients and for splats without gradient;
+    // adding: the used elements of splats with one, the rest left as they are)
+    const uint64_t wb = __ballot(valid && (assign || seen != 0));
+    float4* __restrict__ gsh4w = reinterpret_cast<float4*>(out.sh);
+#pragma unroll
+    for (int ph = 0; ph < 4; ++ph) {
+      if (((wb >> (16 * ph)) & 0xFFFFull) == 0ull) continue;  // uniform
+      if ((lane >> 4) == ph) {
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+          shw[(lane & 15) * kBwdShPad + q] =
+              make_float4(gsh[4 * q], gsh[4 * q + 1], gsh[4 * q + 2], gsh[4 * q + 3]);
+      }
+      wave_sync();
+#pragma unroll
+      for (int kq = 0; kq < 3; ++kq) {
+        const int idx = kq * 64 + lane, gl = idx / 12, q = idx - gl * 12;
+        if (!((wb >> (16 * ph + gl)) & 1ull)) continue;
+        const float4 gvq = shw[gl * kBwdShPad + q];
+        float4* dst = gsh4w + (size_t)(wfirst + 16u * ph + gl) * 12u + q;
+        if (assign) {
+          *dst = gvq;
+        } else if (4 * q < nfloat) {
+          float4 o = *dst;
+          if (4 * q + 0 < nfloat) o.x += gvq.x;
+          if (4 * q + 1 < nfloat) o.y += gvq.y;
+          if (4 * q + 2 < nfloat) o.z += gvq.z;
+          if (4 * q + 3 < nfloat) o.w += gvq.w;
+          *dst = o;
+        }
+      }
+      wave_sync();
+    }
+  }
+  if (!valid) return;
   float4* gsh4 = reinterpret_cast<float4*>(out.sh) + (size_t)i * 12u;
   if (!seen) {  // no gradient; written as zero when assigning
     if (assign) {
       for (int k = 0; k < 11; ++k) out.p[k][i] = 0.0f;
+      if (!SINGLE) {
 #pragma unroll
-      for (int qd = 0; qd < 12; ++qd) gsh4[qd] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int qd = 0; qd < 12; ++qd) gsh4[qd] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
     }
     return;
   }
@@ -710,14 +811,14 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(SceneDev s,
   if (out.count) out.count[i] += seen;
   // dL/dsh as float4s (assigning: every element, zero past the used
   // coefficients; adding: those elements are left as they are)
-  if (assign) {
+  if (assign && !SINGLE) {
 #pragma unroll
     for (int qd = 0; qd < 12; ++qd)
       gsh4[qd] = make_float4(gsh[4 * qd], gsh[4 * qd + 1], gsh[4 * qd + 2], gsh[4 * qd + 3]);
   }
 #pragma unroll
   for (int qd = 0; qd < 12; ++qd) {
-    if (assign || 4 * qd >= nfloat) continue;
+    if (SINGLE || assign || 4 * qd >= nfloat) continue;
     float4 o = gsh4[qd];
     if (4 * qd + 0 < nfloat) o.x += gsh[4 * qd + 0];
     if (4 * qd + 1 < nfloat) o.y += gsh[4 * qd + 1];
@@ -855,7 +956,8 @@ static LumenStatus backward_impl(LumenExContext* c, const LumenExScene* sc,
   hipLaunchKernelGGL(k_sum_fixup, dim3(div_up(v_cap, 256)), dim3(256), 0, st, vmeta, plan, aux,
                      gv);
   LUMEN_LAUNCHED();
-  hipLaunchKernelGGL(k_preprocess_bwd, dim3(div_up(sc->count, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(nv == 1 ? k_preprocess_bwd<true> : k_preprocess_bwd<false>,
+                     dim3(div_up(sc->count, 256)), dim3(256), 0, st,
                      scene_dev(sc), views, nv, c->packed.as<uint64_t>(),
                      c->totals.as<uint64_t>(), plan, gv, gp,
                      (cfg->flags & LUMEN_EX_GRAD_ASSIGN) != 0);
