@@ -39,8 +39,8 @@ stage with the largest share; its algorithmic work per launch is:
              splat (geometry cached for emit); + 4 B / Gaussian when emit
              runs the first sort pass (its digit histogram, 512 words per
              512 Gaussians)
-  emit       24 B / Gaussian (position, opacity, scan offset) + 296 B / visible
-             splat (32 B cached geometry + 192 B SH read, 48 B record + 24 B
+  emit       24 B / Gaussian (position, opacity, scan offset) + 312 B / visible
+             splat (32 B cached geometry + 192 B SH read, 64 B record + 24 B
              index, radius and key metadata written) + 8 B / key written
              (+ 4 B / Gaussian of digit prefixes read when it runs the first
              sort pass: it then writes the keys in that pass's order)
@@ -51,7 +51,7 @@ stage with the largest share; its algorithmic work per launch is:
   composite  30 FP32 operations per (pixel, splat) evaluation of the
              reference's compositor (E of them per frame, counted by the
              oracle), the scope table's fixed charge -- VALU bound; its bytes
-             are 52 B / key + 8 B / tile + 16 B / pixel
+             are 68 B / key + 8 B / tile + 16 B / pixel
 `traffic` is the HBM bytes per launch rocprofv3 counted for that kernel
 (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a plain copy of
 known size), read from the committed summary under profiles/ when present.
@@ -102,7 +102,7 @@ def stage_work(N, V, M, T, passes, E, kb, emit_passes=0):
     emit_passes of the passes ran in emit."""
     return {
         "count": ("hbm", 68.0 * N + 32.0 * V + 4.0 * N * emit_passes),
-        "emit": ("hbm", 24.0 * N + 296.0 * V + (kb + 4.0) * M + 4.0 * N * emit_passes),
+        "emit": ("hbm", 24.0 * N + 312.0 * V + (kb + 4.0) * M + 4.0 * N * emit_passes),
         # per pass the sort runs: the histogram reads the keys, the scatter
         # reads and writes keys and values
         "sort": ("hbm", (passes - emit_passes) * (kb + 2.0 * (kb + 4.0)) * M),
