@@ -956,7 +956,10 @@ __global__ __launch_bounds__(1024) void k_tile_order(const int2* __restrict__ ra
   }
 }
 
-template <int BATCH, int PIX>
+// SAVE: the frame keeps each pixel's final transmittance and evaluation
+// count (for the backward); without it the blend does not track where each
+// pixel terminated.
+template <int BATCH, int PIX, bool SAVE>
 __global__ __launch_bounds__(256) void k_composite(
     const int2* __restrict__ ranges, const uint32_t* __restrict__ v0,
     const uint32_t* __restrict__ v1, const SortPlan* __restrict__ plan,
@@ -1064,11 +1067,12 @@ __global__ __launch_bounds__(256) void k_composite(
             C2[p] += c.z * wgt;
             D[p] += c.w * wgt;
             T[p] *= (1.0f - alpha);
-            ne[p] = T[p] <= kTransmittanceEps ? base + j - rg.x + 1 : ne[p];
+            if (SAVE) ne[p] = T[p] <= kTransmittanceEps ? base + j - rg.x + 1 : ne[p];
           }
+          // (a bitwise or of the compares: an or of maxima canonicalizes NaNs)
           bool alive = false;
 #pragma unroll
-          for (int p = 0; p < PIX; ++p) alive |= T[p] > kTransmittanceEps;
+          for (int p = 0; p < PIX; ++p) alive = alive | (T[p] > kTransmittanceEps);
           if (__ballot(alive) == 0) break;
         }
         any = false;
@@ -1090,8 +1094,10 @@ __global__ __launch_bounds__(256) void k_composite(
     rgb[3 * pix + 1] = C1[p] + bg;
     rgb[3 * pix + 2] = C2[p] + bg;
     if (depth) depth[pix] = v.compute_depth ? D[p] : 0.0f;
-    if (t_final) t_final[pix] = T[p];
-    if (n_eval) n_eval[pix] = ne[p];
+    if (SAVE) {
+      t_final[pix] = T[p];
+      n_eval[pix] = ne[p];
+    }
   }
 }
 
@@ -1291,9 +1297,11 @@ static LumenStatus composite_launch(uint32_t batch, uint64_t tiles, uint32_t til
   // two pixels per lane where the tile has at least 128 pixels
   const bool two = threads % 128 == 0;
   const dim3 grid((uint32_t)tiles), block(two ? threads / 2 : threads);
-#define LUMEN_COMPOSITE(B, P)                                                                  \
-  hipLaunchKernelGGL((k_composite<B, P>), grid, block, 0, st, ranges, v0, v1, plan, rec, views, \
-                     (int)nviews, spans, rgb, depth, t_final, n_eval, order)
+  const bool save = t_final != nullptr && n_eval != nullptr;
+#define LUMEN_COMPOSITE(B, P)                                                                   \
+  hipLaunchKernelGGL((save ? k_composite<B, P, true> : k_composite<B, P, false>), grid, block, 0, \
+                     st, ranges, v0, v1, plan, rec, views, (int)nviews, spans, rgb, depth,       \
+                     t_final, n_eval, order)
   if (two) {
     if (batch == 64) LUMEN_COMPOSITE(64, 2);
     else if (batch == 128) LUMEN_COMPOSITE(128, 2);

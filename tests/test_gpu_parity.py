@@ -89,6 +89,12 @@ def compare(rasterizer, oracle, scene, camera, config=None, exact_images=True):
                                       ref["n_eval"])
     else:
         np.testing.assert_array_equal(rgb2, rgb)
+    # and without the per-pixel state (the compositor instance that does not
+    # track where pixels terminate): the same pixels
+    out3 = rasterizer.render(scene, camera, config)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out3.rgb.cpu().numpy(), rgb2)
+    np.testing.assert_array_equal(out3.depth.cpu().numpy(), out2.depth.cpu().numpy())
     return ref, out
 
 
