@@ -82,6 +82,40 @@ def test_graph_replays_match_eager(renderer):
         del junk
 
 
+def test_single_view_graph_with_sort_pass_in_emit(renderer, oracle):
+    """A one-view frame of 3600 tiles captured as a graph: the key emission
+    runs the sort's first pass inside the graph (digit table, column scans
+    in the planning kernel).  Replays, with eager work and synchronizations
+    in between, equal the eager frame and the oracle; a replay whose scene
+    outgrew the reserved keys reports the capacity error without faulting."""
+    import torch
+    from lumen_amd._lib import LumenError
+    scene = random_scene(100_000, 1.0, 95)
+    dev = scene.to_device()
+    cam = look_at(CameraConfig(1280, 720, 1000.0, 1000.0, 640.0, 360.0), (0.4, 0.3, 5.0))
+    cfg = RenderConfig()
+    eager = renderer.render(dev, [cam], cfg, graph=False)
+    renderer.synchronize()
+    ref_rgb = eager.rgb.clone()
+    ref = oracle.rasterize(scene, cam, cfg)
+    np.testing.assert_array_equal(ref_rgb.cpu().numpy().reshape(ref["rgb"].shape), ref["rgb"])
+    for rep in range(3):
+        say(f"[batch] single-view graph replay {rep}")
+        out = renderer.render(dev, [cam], cfg, graph=True)
+        renderer.status(1)
+        junk = torch.randn(1000, 1000, device="cuda") @ torch.randn(1000, 1000, device="cuda")
+        torch.cuda.synchronize()
+        assert torch.equal(out.rgb, ref_rgb), rep
+        del junk
+    dev.planes[3:6] *= 8.0
+    out = renderer.render(dev, [cam], cfg, graph=True)
+    with pytest.raises(LumenError) as ei:
+        renderer.status(1)
+    assert ei.value.status == -101
+    renderer.synchronize()
+    assert float(out.rgb.abs().max()) == 0.0
+
+
 def test_graph_capacity_overflow_is_reported(renderer):
     """A replay whose scene grew past the reserved key capacity reports
     LUMEN_EX_ERROR_CAPACITY and writes background, without faulting."""
