@@ -116,6 +116,57 @@ def test_single_view_graph_with_sort_pass_in_emit(renderer, oracle):
     assert float(out.rgb.abs().max()) == 0.0
 
 
+def test_batch_sorted_stages_without_kept_keys(renderer, rasterizer, oracle):
+    """Three 1280x720 views (3600 tiles each) in one frame that does not keep
+    its unsorted keys: each view's sorted keys, values and tile ranges equal a
+    single-view render that kept them, as do the images; graph replays equal
+    the eager frame, and one view equals the oracle."""
+    import torch
+    scene = random_scene(100_000, 1.0, 96)
+    dev = scene.to_device()
+    cfg = RenderConfig()
+    ccfg = CameraConfig(1280, 720, 1000.0, 1000.0, 640.0, 360.0)
+    cams = [look_at(ccfg, eye) for eye in ((0.4, 0.3, 5.0), (4.0, 1.0, 2.5), (-3.0, -0.5, -3.8))]
+    say("[batch] 3 views of 1280x720, sorted stages")
+    out = renderer.render(dev, cams, cfg, want_stats=True)
+    renderer.synchronize()
+    st = out.stats
+    M = sum(s.total_splat_instances for s in st)
+    T = 3 * 80 * 45
+    R = renderer.rasterizer
+    keys = R.frame_stage("keys", M, np.uint64)
+    vals = R.frame_stage("vals", M, np.uint32)
+    ranges = R.frame_stage("ranges", 2 * T, np.int32).reshape(T, 2)
+    ref_rgb = out.rgb.clone()
+    tile_base = 0
+    for k, cam in enumerate(cams):
+        one = rasterizer.render(dev, cam, cfg, keep_stages=True)
+        torch.cuda.synchronize()
+        assert one.emit_passes == 0
+        kb, n, vb = st[k].key_base, st[k].total_splat_instances, st[k].vis_base
+        assert n == one.total_splat_instances and n > 0
+        assert torch.equal(out.views_rgb[k], one.rgb), k
+        assert torch.equal(out.views_depth[k], one.depth), k
+        np.testing.assert_array_equal(keys[kb:kb + n] - (np.uint64(tile_base) << np.uint64(32)),
+                                      rasterizer.stage("keys"))
+        np.testing.assert_array_equal(vals[kb:kb + n] - vb, rasterizer.stage("vals"))
+        rb = ranges[tile_base:tile_base + 3600].copy()
+        rb[rb[:, 0] >= 0] -= kb
+        np.testing.assert_array_equal(rb, rasterizer.stage("ranges"))
+        tile_base += 3600
+    ref = oracle.rasterize(scene, cams[1], cfg)
+    np.testing.assert_array_equal(out.views_rgb[1].cpu().numpy().reshape(ref["rgb"].shape),
+                                  ref["rgb"])
+    for rep in range(3):
+        say(f"[batch] 3-view graph replay {rep}")
+        g = renderer.render(dev, cams, cfg, graph=True)
+        renderer.status(3)
+        junk = torch.randn(1000, 1000, device="cuda") @ torch.randn(1000, 1000, device="cuda")
+        torch.cuda.synchronize()
+        assert torch.equal(g.rgb, ref_rgb), rep
+        del junk
+
+
 def test_graph_capacity_overflow_is_reported(renderer):
     """A replay whose scene grew past the reserved key capacity reports
     LUMEN_EX_ERROR_CAPACITY and writes background, without faulting."""
