@@ -1069,11 +1069,14 @@ __global__ __launch_bounds__(256) void k_composite(
             T[p] *= (1.0f - alpha);
             if (SAVE) ne[p] = T[p] <= kTransmittanceEps ? base + j - rg.x + 1 : ne[p];
           }
-          // (a bitwise or of the compares: an or of maxima canonicalizes NaNs)
-          bool alive = false;
+          // T is never negative, so its bits order as the values do: one
+          // unsigned maximum and one compare (a NaN T, from NaN inputs, counts
+          // as live here and keeps the wave walking; its pixel's own test
+          // above skips every splat, as the reference's does)
+          uint32_t tbits = 0u;
 #pragma unroll
-          for (int p = 0; p < PIX; ++p) alive = alive | (T[p] > kTransmittanceEps);
-          if (__ballot(alive) == 0) break;
+          for (int p = 0; p < PIX; ++p) tbits = max(tbits, __float_as_uint(T[p]));
+          if (__ballot(tbits > __float_as_uint(kTransmittanceEps)) == 0) break;
         }
         any = false;
 #pragma unroll
