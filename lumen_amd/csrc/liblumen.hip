@@ -97,6 +97,9 @@ struct GeoCache {
 // INL: the views come in the kernel's arguments (iv) instead of device
 // memory, and block 0 writes them to views_out for the frame's later kernels
 // (a frame of at most kInlineViews views needs no copy of its own).
+// With bsum, each block also writes the sum of its packed words (single-view
+// frames: the block's Gaussians are one run of the scanned array) to
+// bsum[Gaussian block], the partial sums the scan's downsweep starts from.
 // With etab (single-view frames, frame_fusable), each block also writes the
 // histogram of k_emit's digit (depth key & 511) over its Gaussians, weighted
 // by their tile counts: etab[digit * blocks + block], the rows k_emit's first
@@ -119,10 +122,11 @@ __global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* _
                                                uint4* __restrict__ block_bounds,
                                                uint64_t* __restrict__ block_inst, InlineViews iv,
                                                ViewParams* __restrict__ views_out,
-                                               uint32_t* __restrict__ etab) {
+                                               uint32_t* __restrict__ etab,
+                                               uint64_t* __restrict__ bsum) {
   constexpr int NW = BLOCK / 64;
   __shared__ uint32_t sb[NW][4];
-  __shared__ uint64_t sinst[NW];
+  __shared__ uint64_t sinst[NW], spk[NW];
   __shared__ uint32_t eh[kDigitBins];
   const uint32_t gb = xcd_block(blockIdx.x, gridDim.x);  // the block's Gaussians
   const uint32_t i = gb * (uint32_t)BLOCK + threadIdx.x;
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* _
     __syncthreads();
   }
   uint32_t dmin = 0xFFFFFFFFu, dmax = 0u, tmin = 0xFFFFFFFFu, tmax = 0u;
-  uint64_t inst = 0ull;
+  uint64_t inst = 0ull, pk = 0ull;
   if (i < s.n) {
     const float px = s.px[i], py = s.py[i], pz = s.pz[i];
     const float s0 = s.sx[i], s1 = s.sy[i], s2 = s.sz[i];
@@ -164,6 +168,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* _
         geo[idx] = gc;
       }
       packed[idx] = val;
+      pk += val;
     }
   }
   dmin = wave_min_u32_all(dmin);
@@ -171,6 +176,7 @@ __global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* _
   tmin = wave_min_u32_all(tmin);
   tmax = wave_max_u32_all(tmax);
   inst = wave_sum(inst);
+  if (bsum) pk = wave_sum(pk);  // uniform: a kernel argument
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sb[w][0] = dmin;
@@ -178,19 +184,22 @@ __global__ __launch_bounds__(BLOCK) void k_count(SceneDev s, const ViewParams* _
     sb[w][2] = tmin;
     sb[w][3] = tmax;
     sinst[w] = inst;
+    spk[w] = pk;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint64_t bi = sinst[0];
+    uint64_t bi = sinst[0], bp = spk[0];
     for (int k = 1; k < NW; ++k) {
       dmin = min(dmin, sb[k][0]);
       dmax = max(dmax, sb[k][1]);
       tmin = min(tmin, sb[k][2]);
       tmax = max(tmax, sb[k][3]);
       bi += sinst[k];
+      bp += spk[k];
     }
     block_bounds[blockIdx.x] = make_uint4(dmin, dmax, tmin, tmax);
     block_inst[blockIdx.x] = bi;
+    if (bsum) bsum[gb] = bp;
   }
   if (etab) {  // the __syncthreads above ordered the LDS atomics
     for (int e = threadIdx.x; e < kDigitBins; e += BLOCK)
@@ -1229,6 +1238,14 @@ static bool frame_fusable(uint32_t nviews, uint64_t tiles, uint64_t n, bool keep
 // Gaussians per block of k_count and k_emit (the digit table's rows).
 static uint32_t count_block(const FrameShape& fs) { return fs.fusable ? kFusedBlock : 256u; }
 
+// Whether k_count's block sums replace the scan's reduce pass: one view (its
+// blocks are consecutive pieces of the scanned array, and count_block
+// divides the scan's chunk) of at most kScanFuse chunks.
+static bool scan_pieces(const FrameShape& fs, uint64_t n) {
+  static_assert(kScanChunk % kFusedBlock == 0 && kScanChunk % 256 == 0, "pieces of a chunk");
+  return fs.nviews == 1 && n > 0 && div_up(n, (uint64_t)kScanChunk) <= kScanFuse;
+}
+
 // Per-view parameters on the host and their device copy: through pinned
 // staging, ordered on the stream, or (allow_inline, at most kInlineViews
 // views, a frame that runs k_count) written by k_count from its arguments.
@@ -1332,7 +1349,9 @@ static LumenStatus frame_buffers(LumenExContext* c, Need&& need, uint64_t N, con
   LUMEN_TRY(need(c->geo, (nv ? nv : 1) * sizeof(GeoCache)));
   // per count block: bounds (16 B), then after all of them the exact instance sums (8 B)
   LUMEN_TRY(need(c->block_bounds, (uint64_t)div_up(N ? N : 1, 256) * 24));
-  LUMEN_TRY(need(c->part, (uint64_t)div_up(nv ? nv : 1, kScanChunk) * 8));
+  // the scan's chunk sums, or k_count's block sums (scan_pieces)
+  LUMEN_TRY(need(c->part, std::max<uint64_t>(div_up(nv ? nv : 1, kScanChunk),
+                                             div_up(N ? N : 1, 256)) * 8));
   // 8 B totals, the 16 B bounds and the 8 B exact instance total (one read-back)
   LUMEN_TRY(need(c->totals, 32));
   LUMEN_TRY(need(c->plan, sizeof(SortPlan)));
@@ -1405,13 +1424,21 @@ static LumenStatus enqueue_count(LumenExContext* c, const SceneDev& sd, const Fr
   ViewParams* vout = fs.inline_views ? c->views.as<ViewParams>() : nullptr;
   auto count = fs.fusable ? (fs.inline_views ? k_count<true, kFusedBlock> : k_count<false, kFusedBlock>)
                           : (fs.inline_views ? k_count<true, 256> : k_count<false, 256>);
+  const uint64_t nv = (uint64_t)fs.nviews * sd.n;
+  // a single view's blocks are consecutive pieces of the scanned array: their
+  // sums stand in for the scan's reduce pass
+  const bool pieces = scan_pieces(fs, sd.n);
   hipLaunchKernelGGL(count, dim3(nblocks), dim3(cb), 0, st, sd, c->views.as<ViewParams>(),
                      (int)fs.nviews, c->packed.as<uint64_t>(), c->geo.as<GeoCache>(), bb, binst,
-                     iv, vout, etab);
+                     iv, vout, etab, pieces ? c->part.as<uint64_t>() : nullptr);
   LUMEN_LAUNCHED();
-  const uint64_t nv = (uint64_t)fs.nviews * sd.n;
-  LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
-                                     c->part.as<uint64_t>(), c->totals.as<uint64_t>(), st));
+  if (pieces)
+    LUMEN_TRY(exclusive_scan_pieces<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(),
+                                              nv, c->part.as<uint64_t>(), cb,
+                                              c->totals.as<uint64_t>(), st));
+  else
+    LUMEN_TRY(exclusive_scan<uint64_t>(c->packed.as<uint64_t>(), c->packed.as<uint64_t>(), nv,
+                                       c->part.as<uint64_t>(), c->totals.as<uint64_t>(), st));
   LUMEN_TRY(mark(c, LUMEN_EX_TIME_COUNT, true, st));
   return LUMEN_SUCCESS;
 }

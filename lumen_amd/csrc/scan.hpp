@@ -124,7 +124,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* __restrict_
 
 // k_scan_down for at most kScanFuse chunks: each block adds up the partial
 // sums of the chunks before it itself (no spine launch); the last block
-// writes the total.
+// writes the total.  per_chunk partials cover one chunk (1 from
+// k_scan_reduce; more when the producer of the data summed smaller pieces
+// of it, in order, as it wrote them).
 constexpr uint32_t kScanFuse = 1024;
 
 template <typename T>
@@ -132,7 +134,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
                                                                   uint64_t n,
                                                                   const T* __restrict__ partials,
                                                                   T* __restrict__ out,
-                                                                  T* __restrict__ total_out, ScanGate gate) {
+                                                                  T* __restrict__ total_out, ScanGate gate,
+                                                                  uint32_t per_chunk) {
   if (gate_closed(gate)) return;
   __shared__ T s[kScanChunk + kScanChunk / 32];
   __shared__ T s_w[kScanThreads / 64];
@@ -142,12 +145,31 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
   // past the filled length nothing is written, except the total by the last block
   if (base >= n && !(total_out && blockIdx.x == gridDim.x - 1)) return;
   const int t = threadIdx.x;
+  // the chunk's elements, loaded together with the partials before it
+  T x[kScanItems];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const uint64_t idx = base + (uint64_t)(j * kScanThreads + t);
+    x[j] = idx < n ? in[idx] : T(0);
+  }
   {
+    const uint32_t lim = blockIdx.x * per_chunk;
     T p = 0;
-    for (uint32_t b = t; b < blockIdx.x; b += kScanThreads) p += partials[b];
+    for (uint32_t b0 = 0; b0 < lim; b0 += 4u * kScanThreads) {  // four loads in flight
+      T q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t b = b0 + (uint32_t)(j * kScanThreads + t);
+        q[j] = b < lim ? partials[b] : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) p += q[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) s[scan_pad(j * kScanThreads + t)] = x[j];
     p = wave_sum(p);
     if ((t & 63) == 0) s_w[t >> 6] = p;
-    __syncthreads();
+    __syncthreads();  // (also orders the chunk's LDS writes before the reads below)
     if (t == 0) {
       T q = 0;
 #pragma unroll
@@ -156,13 +178,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down_fused(const T* __res
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int li = j * kScanThreads + t;
-    const uint64_t idx = base + li;
-    s[scan_pad(li)] = idx < n ? in[idx] : T(0);
-  }
-  __syncthreads();
   T v[kScanItems];
   T sum = 0;
 #pragma unroll
@@ -200,7 +215,7 @@ inline LumenStatus exclusive_scan(const T* in, T* out, uint64_t n, T* partials, 
   LUMEN_LAUNCHED();
   if (nb <= kScanFuse) {
     hipLaunchKernelGGL(k_scan_down_fused<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials,
-                       out, total_out, gate);
+                       out, total_out, gate, 1u);
     LUMEN_LAUNCHED();
     return LUMEN_SUCCESS;
   }
@@ -208,6 +223,21 @@ inline LumenStatus exclusive_scan(const T* in, T* out, uint64_t n, T* partials, 
   LUMEN_LAUNCHED();
   hipLaunchKernelGGL(k_scan_down<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials, out,
                      gate);
+  LUMEN_LAUNCHED();
+  return LUMEN_SUCCESS;
+}
+
+// Exclusive scan of n elements (in may equal out) whose producer already
+// wrote the sums of its consecutive pieces of `piece` elements to partials
+// (piece divides kScanChunk; at most kScanFuse chunks): the downsweep alone.
+template <typename T>
+inline LumenStatus exclusive_scan_pieces(const T* in, T* out, uint64_t n, const T* partials,
+                                         uint32_t piece, T* total_out, hipStream_t st) {
+  const uint32_t nb = div_up(n ? n : 1, kScanChunk);
+  if (nb > kScanFuse || piece == 0 || kScanChunk % piece != 0)
+    return fail(LUMEN_ERROR_INVALID_ARGUMENT, "scan: %u chunks, pieces of %u", nb, piece);
+  hipLaunchKernelGGL(k_scan_down_fused<T>, dim3(nb), dim3(kScanThreads), 0, st, in, n, partials,
+                     out, total_out, ScanGate{}, (uint32_t)kScanChunk / piece);
   LUMEN_LAUNCHED();
   return LUMEN_SUCCESS;
 }
