@@ -482,21 +482,25 @@ __device__ __forceinline__ float rect_min_power_rec(float sx, float sy, float i0
 // over the block's pixel centres exceeds the record's cut.  The same answer
 // strip_mask / block_mask give from the splat's raw quantities (full-width
 // strips are blocks of qw = tile width).
-__device__ __forceinline__ uint32_t record_mask(const SplatRecord& r, int tile_x0, int tile_y0,
-                                                int qw, int qh, int nwx, int nwaves) {
+// Whether a record's splat can reach the block of qw x qh pixels at (bx, by)
+// (one wave's bit of record_mask; q0, q1 and q3 of the record are read).
+__device__ __forceinline__ bool record_hits_block(const SplatRecord& r, int bx, int by, int qw,
+                                                  int qh) {
   const float sx = r.q0.x, sy = r.q0.y, hx = r.q3.x, hy = r.q3.y, cut = r.q1.w;
   const float i00 = 2.0f * r.q0.z, i01 = r.q0.w, i11 = 2.0f * r.q1.x;
+  const float x0 = (float)bx + 0.5f, x1 = (float)(bx + qw) - 0.5f;
+  const float y0 = (float)by + 0.5f, y1 = (float)(by + qh) - 0.5f;
+  if (sx + hx < x0 || sx - hx > x1 || sy + hy < y0 || sy - hy > y1) return false;
+  return !(cut < __builtin_inff() &&
+           rect_min_power_rec(sx, sy, i00, i01, i11, r.q3.z, r.q3.w, x0, x1, y0, y1) > cut);
+}
+
+__device__ __forceinline__ uint32_t record_mask(const SplatRecord& r, int tile_x0, int tile_y0,
+                                                int qw, int qh, int nwx, int nwaves) {
   uint32_t m = 0u;
-  for (int w = 0; w < nwaves; ++w) {
-    const int bx = tile_x0 + (w % nwx) * qw, by = tile_y0 + (w / nwx) * qh;
-    const float x0 = (float)bx + 0.5f, x1 = (float)(bx + qw) - 0.5f;
-    const float y0 = (float)by + 0.5f, y1 = (float)(by + qh) - 0.5f;
-    if (sx + hx < x0 || sx - hx > x1 || sy + hy < y0 || sy - hy > y1) continue;
-    if (cut < __builtin_inff() &&
-        rect_min_power_rec(sx, sy, i00, i01, i11, r.q3.z, r.q3.w, x0, x1, y0, y1) > cut)
-      continue;
-    m |= 1u << w;
-  }
+  for (int w = 0; w < nwaves; ++w)
+    if (record_hits_block(r, tile_x0 + (w % nwx) * qw, tile_y0 + (w / nwx) * qh, qw, qh))
+      m |= 1u << w;
   return m;
 }
 
