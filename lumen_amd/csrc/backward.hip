@@ -207,6 +207,10 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // each pixel's last evaluated splat as a position in this batch
+    int lrel[PIX];
+#pragma unroll
+    for (int p = 0; p < PIX; ++p) lrel[p] = last[p] - start;
     for (int q = (int)cnt - 1; q >= 0; --q) {  // back to front; every lane runs every q
       const int j = s_list[wv][q];
       const float4 a = s0[j];  // x, y, i00/2, i01
@@ -216,7 +220,11 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
       const float txx = a.z * dx * dx;
       const float txy = a.w * dx;
       float gv[kGradVis];  // set by the first pixel, added to by the others
-      bool contrib = false;
+      // a blend weight of a pixel that blended this splat: never zero (alpha
+      // >= 1/255, and T >= T_final, which the forward's termination keeps
+      // >= 1e-4 x 0.01), so its compare is the wave's "any contribution"
+      // test, taken straight from the weight's register
+      float wany = 0.0f;
 #pragma unroll
       for (int p = 0; p < PIX; ++p) {
         const float dy = pyc[p] - a.y;
@@ -230,13 +238,12 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
         // splats up to the pixel's last evaluated one, within the power
         // limit (the forward skipped the others too); one branch, the
         // clamps' cases below are selects
-        if (start + j <= last[p] && !(power > b.z)) {
+        if (j <= lrel[p] && !(power > b.z)) {
           const float G = splat_exp_neg(-power);
           float alpha = b.y * G;
           const bool aclamp = alpha > 0.99f;
           if (aclamp) alpha = 0.99f;
           if (!(alpha < kMinAlpha)) {
-            contrib = true;
             const float om = 1.0f - alpha;
             const float rom = __builtin_amdgcn_rcpf(om);
             const float Tj = T[p] * rom;
@@ -249,6 +256,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
             Q[p] += gc * w;
             T[p] = Tj;
             wm = w;
+            wany = w;
             const float g5v = dal * G;
             const float dpmv = -dal * b.y * G;
             g5 = aclamp ? 0.0f : g5v;
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(256) void k_composite_bwd(
 #pragma unroll
         for (int k = 0; k < kGradVis; ++k) gv[k] = p == 0 ? t[k] : gv[k] + t[k];
       }
-      if (__ballot(contrib)) {  // wave-uniform; the whole wave is active here
+      if (__ballot(wany != 0.0f)) {  // wave-uniform; the whole wave is active here
         float* dst = &s_part[wv][j * kGradVis];
         if (DEPTH) {
           float sa, sb, sc;
