@@ -999,12 +999,16 @@ __global__ __launch_bounds__(256) void k_composite(
   // image (0 outside, where nothing is written) and the blend that takes it
   // to kTransmittanceEps or below is the pixel's last.
   float C0[PIX], C1[PIX], C2[PIX], T[PIX], D[PIX];
-  int32_t ne[PIX];  // position of the terminating splat + 1, or the list length
+  // the range position of the pixel's last blended splat (SAVE): a pixel
+  // that terminated blends nothing after the splat that ended it, so its
+  // evaluation count is that position + 1; one that did not gets the list
+  // length
+  int32_t lastb[PIX];
 #pragma unroll
   for (int p = 0; p < PIX; ++p) {
     C0[p] = C1[p] = C2[p] = D[p] = 0.0f;
     T[p] = (px < v.width && py0 + p < v.height) ? 1.0f : 0.0f;
-    ne[p] = rg.y - rg.x;
+    lastb[p] = 0;
   }
   if (rg.x != -1 && rg.x < rg.y) {
     const float pxc = (float)px + 0.5f;
@@ -1076,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_composite(
             C2[p] += c.z * wgt;
             D[p] += c.w * wgt;
             T[p] *= (1.0f - alpha);
-            if (SAVE) ne[p] = T[p] <= kTransmittanceEps ? base + j - rg.x + 1 : ne[p];
+            if (SAVE) lastb[p] = base + j;
           }
           // T is never negative, so its bits order as the values do: one
           // unsigned maximum and one compare (a NaN T, from NaN inputs, counts
@@ -1108,7 +1112,7 @@ __global__ __launch_bounds__(256) void k_composite(
     if (depth) depth[pix] = v.compute_depth ? D[p] : 0.0f;
     if (SAVE) {
       t_final[pix] = T[p];
-      n_eval[pix] = ne[p];
+      n_eval[pix] = T[p] <= kTransmittanceEps ? lastb[p] - rg.x + 1 : rg.y - rg.x;
     }
   }
 }
