@@ -975,10 +975,12 @@ __global__ __launch_bounds__(256) void k_composite(
     const float4* __restrict__ rec, const ViewParams* __restrict__ views, int nviews,
     const ViewSpan* __restrict__ spans, float* __restrict__ rgb, float* __restrict__ depth,
     float* __restrict__ t_final, int32_t* __restrict__ n_eval, const uint32_t* __restrict__ order) {
-  static_assert(BATCH <= 256, "list entries are bytes");
+  static_assert(BATCH * sizeof(float4) <= 65536, "list entries are 16-bit byte offsets");
   __shared__ float4 s0[BATCH], s1[BATCH], s2[BATCH];
   __shared__ uint8_t s_mask[BATCH];
-  __shared__ uint8_t s_list[4 / PIX][BATCH];  // one list per wave
+  // one list per wave, of the staged splats' byte offsets in s0 / s1 / s2
+  // (the loop's LDS reads then need no index arithmetic)
+  __shared__ uint16_t s_list[4 / PIX][BATCH];
   const uint32_t gt = order[blockIdx.x];
   const int vw = view_of_tile(views, nviews, gt);
   const ViewParams& v = views[vw];
@@ -1044,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_composite(
         const int j = c0 + lane;
         const bool hit = j < n && ((s_mask[j] >> w) & 1u);
         const uint64_t bal = __ballot(hit);
-        if (hit) s_list[w][cnt + __popcll(bal & lanemask_lt())] = (uint8_t)j;
+        if (hit) s_list[w][cnt + __popcll(bal & lanemask_lt())] = (uint16_t)(j * sizeof(float4));
         cnt += __popcll(bal);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1055,10 +1057,13 @@ __global__ __launch_bounds__(256) void k_composite(
       // exit is wave-uniform
       if (__ballot(any)) {
         for (uint32_t q = 0; q < cnt; ++q) {
-          const int j = s_list[w][q];
-          const float4 a = s0[j];  // x, y, i00/2, i01
-          const float4 b = s1[j];  // i11/2, opacity, power limit
-          const float4 c = s2[j];  // r, g, b, depth
+          const uint32_t jo = s_list[w][q];
+          auto staged = [jo](const float4* arr) {
+            return *reinterpret_cast<const float4*>(reinterpret_cast<const uint8_t*>(arr) + jo);
+          };
+          const float4 a = staged(s0);  // x, y, i00/2, i01
+          const float4 b = staged(s1);  // i11/2, opacity, power limit
+          const float4 c = staged(s2);  // r, g, b, depth
           // compute_gaussian_pixel_alpha, compositor.mojo:62-89
           const float dx = pxc - a.x;
           const float txx = a.z * dx * dx;
@@ -1080,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_composite(
             C2[p] += c.z * wgt;
             D[p] += c.w * wgt;
             T[p] *= (1.0f - alpha);
-            if (SAVE) lastb[p] = base + j;
+            if (SAVE) lastb[p] = base + (int)(jo / sizeof(float4));
           }
           // T is never negative, so its bits order as the values do: one
           // unsigned maximum and one compare (a NaN T, from NaN inputs, counts
