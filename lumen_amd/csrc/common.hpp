@@ -115,12 +115,21 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t x, uint32_t identity) {
   return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)x, CTRL, ROW_MASK, 0xF, false);
 }
 
+// row_shr by CTRL - 0x110 lanes within each 16-lane row, out-of-range lanes
+// reading 0 (bound_ctrl: no old value to keep, so none need be set up).
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr_zero(float x) {
+  static_assert(CTRL > 0x110 && CTRL < 0x120, "row_shr");
+  return __uint_as_float(
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)__float_as_uint(x), CTRL, 0xF, 0xF, true));
+}
+
 // Sum over the 64 lanes; the total is valid in lane 63.
 __device__ __forceinline__ float wave_sum_lane63(float x) {
-  x += __uint_as_float(dpp_u32<0x111, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x112, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x114, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x118, 0xF>(__float_as_uint(x), 0u));
+  x += dpp_shr_zero<0x111>(x);
+  x += dpp_shr_zero<0x112>(x);
+  x += dpp_shr_zero<0x114>(x);
+  x += dpp_shr_zero<0x118>(x);
   x += __uint_as_float(dpp_u32<0x142, 0xA>(__float_as_uint(x), 0u));
   x += __uint_as_float(dpp_u32<0x143, 0xC>(__float_as_uint(x), 0u));
   return x;
@@ -128,10 +137,10 @@ __device__ __forceinline__ float wave_sum_lane63(float x) {
 
 // Sum over each 16-lane row; the row's total is valid in its lane 15.
 __device__ __forceinline__ float row_sum_lane15(float x) {
-  x += __uint_as_float(dpp_u32<0x111, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x112, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x114, 0xF>(__float_as_uint(x), 0u));
-  x += __uint_as_float(dpp_u32<0x118, 0xF>(__float_as_uint(x), 0u));
+  x += dpp_shr_zero<0x111>(x);
+  x += dpp_shr_zero<0x112>(x);
+  x += dpp_shr_zero<0x114>(x);
+  x += dpp_shr_zero<0x118>(x);
   return x;
 }
 
