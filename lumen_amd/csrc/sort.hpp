@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const K* k0, const K
   for (int j = 0; j < ITEMS; ++j) {
     const bool valid = base + j * kSortThreads + t < end;
     const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
-    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1, 64);
+    // the previous lane's digit by a DPP wave shift (lane 0's is not used)
+    const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)d, 0x138, 0xF, 0xF, false);
     const bool head = valid && (lane == 0 || prev != d);
     const uint64_t heads = __ballot(head);
     const uint32_t nvalid = (uint32_t)__popcll(__ballot(valid));  // valid lanes: 0 .. nvalid - 1
