@@ -548,15 +548,28 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
       cnt = (uint32_t)(off1 >> 32) - ko;
     }
     uint32_t rmin = 0, rmax = 0, dk = 0;
+    // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
+    // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139:
+    // 0.5 + the coefficients times the basis, each channel summed in
+    // coefficient order
+    float cr = 0.5f, cg = 0.5f, cb = 0.5f;
     if (!MULTI) {
       // The wave's SH coefficients, loaded together 16 Gaussians at a time:
       // their 3 KB are contiguous, so each load instruction reads 1 KB of
       // consecutive addresses (a lane loading its own 192 B at a 192 B
       // stride meets a different cache line in every lane), and pass through
       // LDS (rows padded to 13 quads, so the 16 readers' quads fall in
-      // distinct banks) to the lanes that own them.  The next group's loads
-      // are issued before this group's exchange.  Only visible splats'
-      // quads are loaded.
+      // distinct banks) to the lanes that own them, which add them into the
+      // colour as they read them (the basis is ready before the loads, so
+      // no lane holds its 48 coefficients).  The next group's loads are
+      // issued before this group's exchange.  Only visible splats' quads
+      // are loaded.
+      float Y[16];
+      {
+        float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
+        normalize3(dx, dy, dz);
+        sh_basis(dx, dy, dz, Y);
+      }
       const uint64_t visb = __ballot(vis != 0u);
       const int nq = (3 * v.sh_coeffs + 3) / 4;  // quads of coefficients the degree uses
       const uint32_t wfirst = first + (uint32_t)(w * 64);
@@ -587,11 +600,19 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
         if ((lane >> 4) == ph && vis) {
 #pragma unroll
           for (int q = 0; q < 12; ++q) {
-            const float4 x = q < nq ? shw[(lane & 15) * kShPad + q] : make_float4(0.f, 0.f, 0.f, 0.f);
-            c[4 * q + 0] = x.x;
-            c[4 * q + 1] = x.y;
-            c[4 * q + 2] = x.z;
-            c[4 * q + 3] = x.w;
+            if (q < nq) {
+              const float4 x = shw[(lane & 15) * kShPad + q];
+              const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const int k = (4 * q + e) / 3, ch = (4 * q + e) % 3;  // constants once unrolled
+                if (k < v.sh_coeffs) {
+                  if (ch == 0) cr += xs[e] * Y[k];
+                  else if (ch == 1) cg += xs[e] * Y[k];
+                  else cb += xs[e] * Y[k];
+                }
+              }
+            }
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -621,19 +642,19 @@ __global__ __launch_bounds__(BLOCK) void k_emit(SceneDev s, const ViewParams* __
         }
         have_sh = true;
       }
-      // evaluate_sh_color(sh, i*48, normalize(pos - eye), degree)
-      // src/math/spherical_harmonics.mojo:133-172, pipeline.mojo:133-139
-      float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
-      normalize3(dx, dy, dz);
-      float Y[16];
-      sh_basis(dx, dy, dz, Y);
-      float cr = 0.5f, cg = 0.5f, cb = 0.5f;
+      if (MULTI) {  // (the single view's colour was summed as its coefficients arrived)
+        float dx = px - v.eye[0], dy = py - v.eye[1], dz = pz - v.eye[2];
+        normalize3(dx, dy, dz);
+        float Y[16];
+        sh_basis(dx, dy, dz, Y);
+        cr = cg = cb = 0.5f;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        if (k < v.sh_coeffs) {
-          cr += c[3 * k + 0] * Y[k];
-          cg += c[3 * k + 1] * Y[k];
-          cb += c[3 * k + 2] * Y[k];
+        for (int k = 0; k < 16; ++k) {
+          if (k < v.sh_coeffs) {
+            cr += c[3 * k + 0] * Y[k];
+            cg += c[3 * k + 1] * Y[k];
+            cb += c[3 * k + 2] * Y[k];
+          }
         }
       }
       if (cr < 0.0f) cr = 0.0f; else if (cr > 1.0f) cr = 1.0f;
